@@ -1,0 +1,165 @@
+/*
+ * sarf.h -- C ABI of the MI355X (gfx950) SAR focusing engine (libsarf.so).
+ *
+ * Plain C: no C++ types cross this boundary, only pointers, sizes and small
+ * POD descriptors.  Every entry point returns SARF_OK (0) on success or a
+ * negative SARF_ERR_* code; the message of the last failure on the calling
+ * thread is returned by sarf_last_error().  No entry point prints, exits or
+ * resets the device.
+ *
+ * Memory: "device" pointers are caller-owned HIP device allocations; work is
+ * enqueued on the caller's stream (hipStream_t passed as void*, NULL = the
+ * null stream) and is complete only after the caller synchronises that
+ * stream.  "host" pointers are ordinary host memory; those entry points
+ * synchronise before returning.
+ *
+ * Layouts (complex = interleaved re,im):
+ *   phase history  ph[n*K + k]        K frequency samples per pulse, pulse n
+ *   range profile  prof[n*Nfft + m]   bin m <-> range r_m = (m/Nfft - 0.5)*maxWr
+ *   antenna        ant[3*n + {0,1,2}] phase-centre x,y,z of pulse n (m), FP64
+ *   image          img[x*Ny + y]      Ny x Nx column-major (y fastest), pixel
+ *                                     (x0 - Wx/2 + x*Wx/(Nx-1), y0 - Wy/2 + y*Wy/(Ny-1), 0)
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the reference repository, charlotte-visionlab/uncc_sar_focusing).
+ */
+#ifndef SARF_H
+#define SARF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SARF_VERSION 100
+
+/* status codes */
+#define SARF_OK 0
+#define SARF_ERR_INVALID (-1)     /* bad argument or descriptor */
+#define SARF_ERR_UNSUPPORTED (-2) /* valid but not supported (e.g. Nfft > 16384) */
+#define SARF_ERR_HIP (-3)         /* HIP runtime error */
+#define SARF_ERR_NOMEM (-4)       /* device allocation failed */
+#define SARF_ERR_NODEVICE (-5)    /* no gfx950 device / bad device index */
+
+/* precision of backprojection */
+#define SARF_BP_MIXED 0 /* FP64 per (tile,pulse) geometry, FP32 per pixel (default) */
+#define SARF_BP_F64 1   /* all-FP64 per pixel and pulse, FP64 profiles and image */
+
+/* backprojection flags */
+#define SARF_BP_ACCUMULATE 1 /* add into img instead of overwriting it */
+
+/* range-compression flags */
+#define SARF_RC_TAPER 1 /* multiply sample k by taper[k] (rangeCompress.m:6-12) */
+#define SARF_RC_RAMP 2  /* multiply pulse n by exp(-j 4 pi f0[n] R0[n] / c) (rangeCompress.m:17) */
+#define SARF_RC_F64 4   /* FP64 in/out (ph: double2, prof: double2) */
+
+typedef struct sarf_ctx sarf_ctx;
+
+/* Library version (SARF_VERSION of the build). */
+int sarf_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *sarf_last_error(void);
+
+/* Create a context bound to HIP device `device` (must be gfx950).
+ * Replaces initialize_GPUMATLAB / initialize_CUDAResources
+ * (src/gpuBackProjection/cuda_sar_focusing/cuda_sar_focusing.hpp:403-478),
+ * without the per-call cudaDeviceReset of finalize_CUDAResources (:482-519).
+ * A context may be used by one host thread at a time. */
+int sarf_ctx_create(int device, sarf_ctx **out);
+int sarf_ctx_destroy(sarf_ctx *ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Range compression: prof[n] = fftshift(ifft(ph[n] zero-padded to Nfft))   */
+/* scaled by 1/Nfft.  Replaces cuifft + cufftNormalize_1DBatch +            */
+/* cufftShift_1DBatch (cuda_sar_focusing.hpp:372-399,616-621;               */
+/* cuda_BackProjectionKernels.cuh:95-135) and the per-pulse ifftw/fftshift  */
+/* of run_bp (src/cpuBackProjection/cpuBackProjection.hpp:152-158), with    */
+/* the zero padding of matlab/bpBasic.m:82.                                 */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int K;     /* frequency samples per pulse, 1 <= K <= Nfft */
+    int Np;    /* pulses, >= 1 */
+    int Nfft;  /* power of two, 32 <= Nfft <= 16384 (FP64: <= 65536) */
+    int flags; /* SARF_RC_* */
+} sarf_rc_desc;
+
+int sarf_range_compress(sarf_ctx *ctx, const sarf_rc_desc *desc,
+                        const void *ph,       /* device, complex K x Np */
+                        const float *taper,   /* device, K, or NULL */
+                        const double *f0,     /* device, Np (SARF_RC_RAMP) or NULL */
+                        const double *r0,     /* device, Np (SARF_RC_RAMP) or NULL */
+                        void *prof,           /* device, complex Nfft x Np */
+                        void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Backprojection of range profiles onto the z = 0 pixel grid.              */
+/* Replaces backprojection_loop (cuda_BackProjectionKernels.cuh:33-93) and  */
+/* computeDifferentialRangeAndPhaseCorrections + interp1                    */
+/* (cpuBackProjection.hpp:24-93):                                           */
+/*   dR = |a_n - p| - R0_n;  if r_0 < dR < r_{Nfft-1}:                      */
+/*   img[p] += lerp(prof[n], dR) * exp(j 4 pi f0_n dR / c)                  */
+/* with the CPU reference's interpolation rule (left bin = floor).          */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int Nx, Ny;   /* image size, >= 2 each */
+    int Nfft;     /* profile length (>= 2) */
+    int Np;       /* pulses */
+    double x0, y0;  /* image centre (m) */
+    double Wx, Wy;  /* image extent (m), spacing W/(N-1) */
+    double maxWr;   /* unambiguous range c/(2 mean deltaF) = max_Wy_m */
+    int precision;  /* SARF_BP_MIXED | SARF_BP_F64 */
+    int flags;      /* SARF_BP_ACCUMULATE */
+    int tile;       /* 0 = autotuned; else a SARF tile id (sarf_autotune) */
+    int pulse_chunk;/* 0 = autotuned; else 8, 16 or 32 */
+} sarf_bp_desc;
+
+int sarf_backproject(sarf_ctx *ctx, const sarf_bp_desc *desc,
+                     const void *prof,   /* device, complex Nfft x Np (float2; double2 for F64) */
+                     const double *ant,  /* device, Np x 3 */
+                     const double *r0,   /* device, Np */
+                     const double *f0,   /* device, Np (start frequency of each pulse) */
+                     void *img,          /* device, complex Ny x Nx (float2; double2 for F64) */
+                     void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Whole focusing call on host buffers: H2D, range compression,             */
+/* backprojection, D2H.  This is what cuda_focus_SAR_image<float,float>     */
+/* (cuda_sar_focusing.hpp:526-723) does; it is the entry the C++ host       */
+/* wrappers in include/uncc_sar_focusing.hpp call.  Synchronous.            */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int K, Np, Nfft;
+    int Nx, Ny;
+    double x0, y0, Wx, Wy, maxWr;
+    int precision; /* SARF_BP_MIXED | SARF_BP_F64 (F64: ph/img stay float on the host) */
+} sarf_focus_desc;
+
+int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *desc,
+               const float *ph,    /* host, complex K x Np */
+               const double *ant,  /* host, Np x 3 */
+               const double *r0,   /* host, Np */
+               const double *f0,   /* host, Np */
+               float *img);        /* host, complex Ny x Nx */
+
+/* ------------------------------------------------------------------------ */
+/* Point-scatterer phase-history synthesis (test/benchmark data source;     */
+/* data model of matlab/formImagePointTargetBP.m:63-81):                    */
+/*   ph[n,k] += sum_s A_s exp(-j 4 pi (f0_n + k df)(|a_n - p_s| - R0_n)/c)  */
+/* Phase argument and its reduction in FP64, rotation in FP32, sums in FP32.*/
+/* Adds into ph (caller zeroes it first).                                   */
+/* ------------------------------------------------------------------------ */
+int sarf_synth_phase_history(sarf_ctx *ctx, int K, int Np, int S,
+                             const double *scat,  /* device, S x 3 */
+                             const float *amp,    /* device, complex S */
+                             const double *ant, const double *r0, const double *f0,
+                             double df, void *ph, /* device, float2 K x Np */
+                             void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SARF_H */

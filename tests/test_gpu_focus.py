@@ -1,0 +1,202 @@
+"""GPU parity of range compression and backprojection against the oracle.
+
+Tolerances (stated per test): mixed-precision BP <= 1e-4 relative L2 against
+the FP64 oracle with identical per-target peak pixels; FP32 range
+compression <= 2e-6 relative L2; FP64 kernels <= 1e-10.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from scene_data import c0_inputs, local_peaks, peak_index, small_inputs
+from uncc_sar_focusing_amd import _lib
+from uncc_sar_focusing_amd.scenes import ImageGrid, circular_aperture
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def to_dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(DEV)
+
+
+def geom_dev(geom):
+    return (to_dev(geom.ant, torch.float64), to_dev(geom.r0, torch.float64),
+            to_dev(geom.f0, torch.float64))
+
+
+# ------------------------------------------------------- range compression
+@pytest.mark.parametrize("nfft", [32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384])
+def test_rc_all_sizes_match_oracle(ctx, nfft):
+    rng = np.random.default_rng(nfft)
+    Np = 9
+    for K in sorted({1, 3, min(nfft, 424), nfft // 2 + 1, nfft}):
+        ph = (rng.standard_normal((Np, K)) + 1j * rng.standard_normal((Np, K))).astype(np.complex64)
+        prof = ctx.range_compress(to_dev(ph), nfft).cpu().numpy()
+        ref = oracle.range_compress_f64(ph, nfft)
+        e = oracle.rel_l2(prof, ref)
+        assert e < 2e-6, (nfft, K, e)
+
+
+def test_rc_taper_and_ramp(ctx):
+    rng = np.random.default_rng(5)
+    Np, K, nfft = 6, 100, 1024
+    ph = (rng.standard_normal((Np, K)) + 1j * rng.standard_normal((Np, K))).astype(np.complex64)
+    taper = np.hamming(K).astype(np.float32)
+    f0 = 9.6e9 + 1e6 * np.arange(Np)
+    r0 = 12500.0 + 0.37 * np.arange(Np)
+    prof = ctx.range_compress(to_dev(ph), nfft, taper=to_dev(taper),
+                              ramp=(to_dev(f0), to_dev(r0))).cpu().numpy()
+    ref = oracle.range_compress_f64(ph, nfft, taper=taper, ramp=(f0, r0))
+    assert oracle.rel_l2(prof, ref) < 2e-6
+
+
+@pytest.mark.parametrize("nfft", [2, 8, 1024, 16384, 32768])
+def test_rc_f64_matches_oracle(ctx, nfft):
+    rng = np.random.default_rng(nfft)
+    Np, K = 5, max(1, nfft // 3)
+    ph = rng.standard_normal((Np, K)) + 1j * rng.standard_normal((Np, K))
+    prof = ctx.range_compress(to_dev(ph), nfft).cpu().numpy()
+    assert oracle.rel_l2(prof, oracle.range_compress_f64(ph, nfft)) < 1e-12
+
+
+def test_rc_rejects_bad_sizes(ctx):
+    ph = to_dev(np.zeros((2, 10), np.complex64))
+    with pytest.raises(_lib.SarfError) as e:
+        ctx.range_compress(ph, 24)
+    assert e.value.code == _lib.SARF_ERR_INVALID
+    with pytest.raises(_lib.SarfError) as e:
+        ctx.range_compress(ph, 16)
+    assert e.value.code == _lib.SARF_ERR_UNSUPPORTED
+
+
+# ----------------------------------------------------------- backprojection
+def test_bp_c0_full_matches_fp64_oracle(ctx):
+    """Configuration C0 in full: RC + mixed BP on the GPU vs FP64 oracle.
+    Tolerance: rel-L2 <= 1e-4, identical per-target and global peak pixels."""
+    cfg, geom, tg, grid, ph = c0_inputs()
+    ant, r0, f0 = geom_dev(geom)
+    prof = ctx.range_compress(to_dev(ph), cfg.nfft)
+    img = ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr).cpu().numpy()
+    ref = oracle.bp_f64(oracle.range_compress_f64(ph, cfg.nfft), geom.ant, geom.r0, geom.f0,
+                        grid, geom.maxWr)
+    e = oracle.rel_l2(img, ref)
+    print(f"C0 mixed rel-L2 vs FP64 oracle: {e:.3e}")
+    assert e <= 1e-4
+    assert peak_index(img) == peak_index(ref)
+    assert local_peaks(img, grid, tg.pos) == local_peaks(ref, grid, tg.pos)
+
+
+def test_bp_c0_f64_matches_oracle(ctx):
+    cfg, geom, tg, grid, ph = c0_inputs()
+    ant, r0, f0 = geom_dev(geom)
+    ph64 = to_dev(ph.astype(np.complex128))
+    prof = ctx.range_compress(ph64, cfg.nfft)
+    img = ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr, precision="f64").cpu().numpy()
+    ref = oracle.bp_f64(oracle.range_compress_f64(ph, cfg.nfft), geom.ant, geom.r0, geom.f0,
+                        grid, geom.maxWr)
+    assert oracle.rel_l2(img, ref) < 1e-10
+
+
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5])
+def test_bp_every_tile_shape(ctx, tile):
+    geom, tg, ph = small_inputs()
+    grid = ImageGrid(70, 45, 0.4, -0.3, 7.0, 5.5)  # ragged vs every tile
+    nfft = 1024
+    ant, r0, f0 = geom_dev(geom)
+    prof = ctx.range_compress(to_dev(ph), nfft)
+    img = ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr, tile=tile).cpu().numpy()
+    ref = oracle.bp_f64(oracle.range_compress_f64(ph, nfft), geom.ant, geom.r0, geom.f0, grid,
+                        geom.maxWr)
+    assert oracle.rel_l2(img, ref) <= 1e-4, tile
+    assert local_peaks(img, grid, tg.pos) == local_peaks(ref, grid, tg.pos)
+
+
+@pytest.mark.parametrize("Np", [1, 31, 33, 100])
+def test_bp_pulse_counts(ctx, Np):
+    geom, tg, ph = small_inputs(Np=Np, K=64)
+    grid = ImageGrid.square(40, 0.15)
+    ant, r0, f0 = geom_dev(geom)
+    prof = ctx.range_compress(to_dev(ph), 512)
+    img = ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr).cpu().numpy()
+    ref = oracle.bp_f64(oracle.range_compress_f64(ph, 512), geom.ant, geom.r0, geom.f0, grid,
+                        geom.maxWr)
+    assert oracle.rel_l2(img, ref) <= 1e-4
+
+
+def test_bp_swath_edge_gate(ctx):
+    """Image larger than the unambiguous range window: pixels outside
+    (r_0, r_{N-1}) must stay zero as in the reference (strict gate)."""
+    geom, tg, ph = small_inputs(Np=16, K=32, extent=2.0)
+    # maxWr = c/(2 df) ~ 7.3 m for K = 32, 640 MHz; image 24 m down-range
+    grid = ImageGrid(8, 160, 0.0, 0.0, 1.4, 24.0)
+    ant, r0, f0 = geom_dev(geom)
+    prof = ctx.range_compress(to_dev(ph), 256)
+    img = ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr).cpu().numpy()
+    ref = oracle.bp_f64(oracle.range_compress_f64(ph, 256), geom.ant, geom.r0, geom.f0, grid,
+                        geom.maxWr)
+    zero_ref = ref == 0
+    assert zero_ref.sum() > 100
+    # exact agreement on the gate except pixels within 1e-5 m of the edge
+    mism = (img == 0) != zero_ref
+    assert mism.sum() <= 2
+    assert oracle.rel_l2(img, ref) <= 1e-4
+
+
+def test_bp_accumulate_is_pulse_split_additive(ctx):
+    geom, tg, ph = small_inputs(Np=80, K=64)
+    grid = ImageGrid.square(48, 0.12)
+    ant, r0, f0 = geom_dev(geom)
+    prof = ctx.range_compress(to_dev(ph), 512)
+    full = ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr)
+    part = ctx.backproject(prof[:30].contiguous(), ant[:30].contiguous(), r0[:30].contiguous(),
+                           f0[:30].contiguous(), grid, geom.maxWr)
+    ctx.backproject(prof[30:].contiguous(), ant[30:].contiguous(), r0[30:].contiguous(),
+                    f0[30:].contiguous(), grid, geom.maxWr, out=part, accumulate=True)
+    assert oracle.rel_l2(part.cpu().numpy(), full.cpu().numpy()) < 1e-6
+
+
+def test_bp_is_deterministic(ctx):
+    geom, tg, ph = small_inputs()
+    grid = ImageGrid.square(64, 0.1)
+    ant, r0, f0 = geom_dev(geom)
+    prof = ctx.range_compress(to_dev(ph), 1024)
+    a = ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr).cpu()
+    b = ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr).cpu()
+    assert torch.equal(a, b)
+
+
+def test_bp_rejects_bad_descriptor(ctx):
+    geom, tg, ph = small_inputs()
+    ant, r0, f0 = geom_dev(geom)
+    prof = ctx.range_compress(to_dev(ph), 1024)
+    with pytest.raises(_lib.SarfError) as e:
+        ctx.backproject(prof, ant, r0, f0, ImageGrid(1, 10, 0, 0, 1.0, 1.0), geom.maxWr)
+    assert e.value.code == _lib.SARF_ERR_INVALID
+    with pytest.raises(_lib.SarfError):
+        ctx.backproject(prof, ant, r0, f0, ImageGrid(8, 8, 0, 0, 1.0, 1.0), -1.0)
+
+
+def test_focus_host_matches_oracle(ctx):
+    geom, tg, ph = small_inputs()
+    grid = ImageGrid(50, 60, 0.0, 0.0, 6.0, 7.0)
+    img = ctx.focus_host(ph, geom.ant, geom.r0, geom.f0, grid, 1024, geom.maxWr)
+    ref = oracle.bp_f64(oracle.range_compress_f64(ph, 1024), geom.ant, geom.r0, geom.f0, grid,
+                        geom.maxWr)
+    assert oracle.rel_l2(img, ref) <= 1e-4
+
+
+def test_synth_matches_oracle(ctx):
+    geom = circular_aperture(33, 300)
+    from uncc_sar_focusing_amd.scenes import point_targets
+    tg = point_targets(300, 30.0, seed=1)
+    ant, r0, f0 = geom_dev(geom)
+    ph = ctx.synth_phase_history(ant, r0, f0, geom.df, geom.K, to_dev(tg.pos),
+                                 to_dev(tg.amp.astype(np.complex64))).cpu().numpy()
+    ref = oracle.synth_ph_f64(geom, tg)
+    assert oracle.rel_l2(ph, ref) < 2e-5

@@ -1,0 +1,102 @@
+"""ctypes binding of libsarf.so (the C ABI declared in include/sarf.h).
+
+There is no CPU fallback: importing this module fails loudly if the HIP
+library has not been built, and every call that returns a non-zero status
+raises :class:`SarfError` with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG_DIR, "libsarf.so")
+
+# status codes (sarf.h)
+SARF_OK = 0
+SARF_ERR_INVALID = -1
+SARF_ERR_UNSUPPORTED = -2
+SARF_ERR_HIP = -3
+SARF_ERR_NOMEM = -4
+SARF_ERR_NODEVICE = -5
+
+SARF_BP_MIXED = 0
+SARF_BP_F64 = 1
+SARF_BP_ACCUMULATE = 1
+SARF_RC_TAPER = 1
+SARF_RC_RAMP = 2
+SARF_RC_F64 = 4
+
+
+class SarfError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libsarf error {code}: {msg}")
+        self.code = code
+
+
+class RcDesc(C.Structure):
+    _fields_ = [("K", C.c_int), ("Np", C.c_int), ("Nfft", C.c_int), ("flags", C.c_int)]
+
+
+class BpDesc(C.Structure):
+    _fields_ = [
+        ("Nx", C.c_int), ("Ny", C.c_int), ("Nfft", C.c_int), ("Np", C.c_int),
+        ("x0", C.c_double), ("y0", C.c_double), ("Wx", C.c_double), ("Wy", C.c_double),
+        ("maxWr", C.c_double),
+        ("precision", C.c_int), ("flags", C.c_int), ("tile", C.c_int), ("pulse_chunk", C.c_int),
+    ]
+
+
+class FocusDesc(C.Structure):
+    _fields_ = [
+        ("K", C.c_int), ("Np", C.c_int), ("Nfft", C.c_int), ("Nx", C.c_int), ("Ny", C.c_int),
+        ("x0", C.c_double), ("y0", C.c_double), ("Wx", C.c_double), ("Wy", C.c_double),
+        ("maxWr", C.c_double), ("precision", C.c_int),
+    ]
+
+
+_vp = C.c_void_p
+_i = C.c_int
+_d = C.c_double
+
+# (name, restype, argtypes) for every symbol sarf.h declares
+SIGNATURES = {
+    "sarf_version": (_i, []),
+    "sarf_last_error": (C.c_char_p, []),
+    "sarf_ctx_create": (_i, [_i, C.POINTER(_vp)]),
+    "sarf_ctx_destroy": (_i, [_vp]),
+    "sarf_range_compress": (_i, [_vp, C.POINTER(RcDesc), _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sarf_backproject": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sarf_focus": (_i, [_vp, C.POINTER(FocusDesc), _vp, _vp, _vp, _vp, _vp]),
+    "sarf_synth_phase_history": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp]),
+}
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(
+            f"libsarf.so not found at {path}; build it with __graft_entry__.build() "
+            "(there is no CPU fallback)"
+        )
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIB: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = load()
+    return _LIB
+
+
+def check(code: int) -> None:
+    if code != SARF_OK:
+        msg = lib().sarf_last_error()
+        raise SarfError(code, msg.decode() if msg else "")

@@ -1,0 +1,149 @@
+"""Python host interface of the MI355X SAR focusing engine.
+
+Thin wrappers over the C ABI (include/sarf.h) that take and return torch
+tensors resident on the GPU; torch is used only for device memory and the
+current HIP stream.  Array conventions follow the reference
+(src/cpuBackProjection/cpuBackProjection.hpp:87, charlotte_sar_api.hpp:145-148):
+
+  phase history  (Np, K)     complex64, pulse-major, K samples per pulse
+  range profiles (Np, Nfft)  complex64
+  antenna        (Np, 3)     float64
+  image          (Nx, Ny)    complex64 -- memory layout img[x*Ny + y]
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+from ._lib import BpDesc, FocusDesc, RcDesc, SarfError, check  # noqa: F401
+from .scenes import ImageGrid
+
+_P = C.c_void_p
+
+
+def _ptr(t: torch.Tensor | None) -> _P:
+    return _P(0 if t is None else t.data_ptr())
+
+
+def _need(t: torch.Tensor, dtype: torch.dtype, shape: tuple, name: str) -> None:
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a CUDA (HIP) tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+class Context:
+    """Owns a libsarf context bound to one GPU."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.lib()
+        self.device = device
+        h = C.c_void_p()
+        check(self.lib.sarf_ctx_create(device, C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self.lib.sarf_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self) -> _P:
+        return _P(torch.cuda.current_stream(self.device).cuda_stream)
+
+    # ------------------------------------------------------------------ RC
+    def range_compress(self, ph: torch.Tensor, nfft: int, *, taper: torch.Tensor | None = None,
+                       ramp: tuple[torch.Tensor, torch.Tensor] | None = None,
+                       out: torch.Tensor | None = None) -> torch.Tensor:
+        """fftshift(ifft(ph, nfft)) per pulse (bpBasic.m:82).  complex64 or
+        complex128 (FP64 path)."""
+        Np, K = ph.shape
+        f64 = ph.dtype == torch.complex128
+        cdt = torch.complex128 if f64 else torch.complex64
+        _need(ph, cdt, (Np, K), "ph")
+        flags = _lib.SARF_RC_F64 if f64 else 0
+        if taper is not None:
+            _need(taper, torch.float32, (K,), "taper")
+            flags |= _lib.SARF_RC_TAPER
+        f0 = r0 = None
+        if ramp is not None:
+            f0, r0 = ramp
+            _need(f0, torch.float64, (Np,), "f0")
+            _need(r0, torch.float64, (Np,), "r0")
+            flags |= _lib.SARF_RC_RAMP
+        if out is None:
+            out = torch.empty((Np, nfft), dtype=cdt, device=ph.device)
+        _need(out, cdt, (Np, nfft), "out")
+        d = RcDesc(K, Np, nfft, flags)
+        check(self.lib.sarf_range_compress(self._h, C.byref(d), _ptr(ph), _ptr(taper), _ptr(f0),
+                                           _ptr(r0), _ptr(out), self._stream()))
+        return out
+
+    # ------------------------------------------------------------------ BP
+    def backproject(self, prof: torch.Tensor, ant: torch.Tensor, r0: torch.Tensor,
+                    f0: torch.Tensor, grid: ImageGrid, maxWr: float, *,
+                    precision: str = "mixed", out: torch.Tensor | None = None,
+                    accumulate: bool = False, tile: int = 0) -> torch.Tensor:
+        Np, nfft = prof.shape
+        f64 = precision == "f64"
+        cdt = torch.complex128 if f64 else torch.complex64
+        _need(prof, cdt, (Np, nfft), "prof")
+        _need(ant, torch.float64, (Np, 3), "ant")
+        _need(r0, torch.float64, (Np,), "r0")
+        _need(f0, torch.float64, (Np,), "f0")
+        if out is None:
+            alloc = torch.zeros if accumulate else torch.empty
+            out = alloc((grid.Nx, grid.Ny), dtype=cdt, device=prof.device)
+        _need(out, cdt, (grid.Nx, grid.Ny), "out")
+        d = BpDesc(grid.Nx, grid.Ny, nfft, Np, grid.x0, grid.y0, grid.Wx, grid.Wy, maxWr,
+                   _lib.SARF_BP_F64 if f64 else _lib.SARF_BP_MIXED,
+                   _lib.SARF_BP_ACCUMULATE if accumulate else 0, tile, 0)
+        check(self.lib.sarf_backproject(self._h, C.byref(d), _ptr(prof), _ptr(ant), _ptr(r0),
+                                        _ptr(f0), _ptr(out), self._stream()))
+        return out
+
+    # --------------------------------------------------------------- focus
+    def focus_host(self, ph, ant, r0, f0, grid: ImageGrid, nfft: int, maxWr: float,
+                   precision: str = "mixed"):
+        """Whole RC + BP on host numpy arrays (what cuda_focus_SAR_image does)."""
+        import numpy as np
+
+        ph = np.ascontiguousarray(ph, dtype=np.complex64)
+        Np, K = ph.shape
+        ant = np.ascontiguousarray(ant, dtype=np.float64)
+        r0 = np.ascontiguousarray(r0, dtype=np.float64)
+        f0 = np.ascontiguousarray(f0, dtype=np.float64)
+        img = np.empty((grid.Nx, grid.Ny), dtype=np.complex64)
+        d = FocusDesc(K, Np, nfft, grid.Nx, grid.Ny, grid.x0, grid.y0, grid.Wx, grid.Wy, maxWr,
+                      _lib.SARF_BP_F64 if precision == "f64" else _lib.SARF_BP_MIXED)
+        check(self.lib.sarf_focus(self._h, C.byref(d), ph.ctypes.data_as(_P),
+                                  ant.ctypes.data_as(_P), r0.ctypes.data_as(_P),
+                                  f0.ctypes.data_as(_P), img.ctypes.data_as(_P)))
+        return img
+
+    # --------------------------------------------------------------- synth
+    def synth_phase_history(self, ant: torch.Tensor, r0: torch.Tensor, f0: torch.Tensor,
+                            df: float, K: int, scat: torch.Tensor, amp: torch.Tensor,
+                            out: torch.Tensor | None = None) -> torch.Tensor:
+        Np = ant.shape[0]
+        S = scat.shape[0]
+        _need(scat, torch.float64, (S, 3), "scat")
+        _need(amp, torch.complex64, (S,), "amp")
+        if out is None:
+            out = torch.zeros((Np, K), dtype=torch.complex64, device=ant.device)
+        _need(out, torch.complex64, (Np, K), "out")
+        check(self.lib.sarf_synth_phase_history(self._h, K, Np, S, _ptr(scat), _ptr(amp),
+                                                _ptr(ant), _ptr(r0), _ptr(f0), float(df),
+                                                _ptr(out), self._stream()))
+        return out

@@ -1,0 +1,221 @@
+// sarf_api.hip -- context, error reporting and the C entry points of libsarf.
+//
+// Replaces the reference's GPUMemoryManager / initialize_* / finalize_*
+// host plumbing (src/gpuBackProjection/cuda_sar_focusing/cuda_sar_focusing.hpp:
+// 93-519) with an explicit context: device buffers are caller-owned, work is
+// stream-ordered, errors are returned as codes (never printed), and the device
+// is never reset.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "sarf_internal.hpp"
+
+namespace sarf {
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+void clear_error() { g_last_error.clear(); }
+
+int ilog2_exact(long n) {
+    if (n <= 0 || (n & (n - 1)) != 0) return -1;
+    int l = 0;
+    while ((1L << l) < n) ++l;
+    return l;
+}
+
+int use_device(sarf_ctx *ctx) {
+    if (!ctx) return fail(SARF_ERR_INVALID, "null context");
+    SARF_HIP(hipSetDevice(ctx->device));
+    return SARF_OK;
+}
+
+int scratch_get(sarf_ctx *ctx, const char *name, size_t bytes, void **out) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    Scratch &s = ctx->scratch[name];
+    if (s.bytes < bytes) {
+        if (s.ptr) {
+            // A previous call may still use the old buffer on some stream.
+            SARF_HIP(hipDeviceSynchronize());
+            SARF_HIP(hipFree(s.ptr));
+            s.ptr = nullptr;
+            s.bytes = 0;
+        }
+        hipError_t e = hipMalloc(&s.ptr, bytes);
+        if (e != hipSuccess) {
+            s.ptr = nullptr;
+            return fail(SARF_ERR_NOMEM, "scratch '%s' of %zu bytes: %s", name, bytes,
+                        hipGetErrorString(e));
+        }
+        s.bytes = bytes;
+    }
+    *out = s.ptr;
+    return SARF_OK;
+}
+
+template <typename T2>
+static int make_twiddles(int log2n, T2 **out) {
+    const long n = 1L << log2n;
+    std::vector<T2> h(n);
+    for (long i = 0; i < n; i++) {
+        // e^{+j 2 pi i / N}, evaluated in double and rounded once
+        double a = 2.0 * kPi * (double)i / (double)n;
+        h[i].x = (decltype(h[i].x))std::cos(a);
+        h[i].y = (decltype(h[i].y))std::sin(a);
+    }
+    T2 *d = nullptr;
+    hipError_t e = hipMalloc(&d, sizeof(T2) * n);
+    if (e != hipSuccess) return fail(SARF_ERR_NOMEM, "twiddle table: %s", hipGetErrorString(e));
+    SARF_HIP(hipMemcpy(d, h.data(), sizeof(T2) * n, hipMemcpyHostToDevice));
+    *out = d;
+    return SARF_OK;
+}
+
+int twiddles_f32(sarf_ctx *ctx, int log2n, const float2 **out) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->tw32.find(log2n);
+    if (it == ctx->tw32.end()) {
+        float2 *d = nullptr;
+        SARF_TRY(make_twiddles(log2n, &d));
+        it = ctx->tw32.emplace(log2n, d).first;
+    }
+    *out = it->second;
+    return SARF_OK;
+}
+
+int twiddles_f64(sarf_ctx *ctx, int log2n, const double2 **out) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->tw64.find(log2n);
+    if (it == ctx->tw64.end()) {
+        double2 *d = nullptr;
+        SARF_TRY(make_twiddles(log2n, &d));
+        it = ctx->tw64.emplace(log2n, d).first;
+    }
+    *out = it->second;
+    return SARF_OK;
+}
+
+}  // namespace sarf
+
+using namespace sarf;
+
+extern "C" {
+
+int sarf_version(void) { return SARF_VERSION; }
+
+const char *sarf_last_error(void) { return g_last_error.c_str(); }
+
+int sarf_ctx_create(int device, sarf_ctx **out) {
+    clear_error();
+    if (!out) return fail(SARF_ERR_INVALID, "sarf_ctx_create: out is NULL");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0)
+        return fail(SARF_ERR_NODEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= count)
+        return fail(SARF_ERR_NODEVICE, "device %d out of range [0,%d)", device, count);
+    hipDeviceProp_t prop;
+    SARF_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(SARF_ERR_NODEVICE, "device %d is %s; libsarf is built for gfx950 only",
+                    device, prop.gcnArchName);
+    sarf_ctx *ctx = new sarf_ctx();
+    ctx->device = device;
+    ctx->cu_count = prop.multiProcessorCount;
+    *out = ctx;
+    return SARF_OK;
+}
+
+int sarf_ctx_destroy(sarf_ctx *ctx) {
+    clear_error();
+    if (!ctx) return SARF_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    for (auto &kv : ctx->tw32) (void)hipFree(kv.second);
+    for (auto &kv : ctx->tw64) (void)hipFree(kv.second);
+    for (auto &kv : ctx->scratch)
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    delete ctx;
+    return SARF_OK;
+}
+
+int sarf_range_compress(sarf_ctx *ctx, const sarf_rc_desc *desc, const void *ph,
+                        const float *taper, const double *f0, const double *r0, void *prof,
+                        void *stream) {
+    clear_error();
+    SARF_TRY(use_device(ctx));
+    if (!desc || !ph || !prof) return fail(SARF_ERR_INVALID, "sarf_range_compress: null argument");
+    return launch_range_compress(ctx, desc, ph, taper, f0, r0, prof, (hipStream_t)stream);
+}
+
+int sarf_backproject(sarf_ctx *ctx, const sarf_bp_desc *desc, const void *prof,
+                     const double *ant, const double *r0, const double *f0, void *img,
+                     void *stream) {
+    clear_error();
+    SARF_TRY(use_device(ctx));
+    if (!desc || !prof || !ant || !r0 || !f0 || !img)
+        return fail(SARF_ERR_INVALID, "sarf_backproject: null argument");
+    return launch_backproject(ctx, desc, prof, ant, r0, f0, img, (hipStream_t)stream);
+}
+
+int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const double *ant,
+               const double *r0, const double *f0, float *img) {
+    clear_error();
+    SARF_TRY(use_device(ctx));
+    if (!d || !ph || !ant || !r0 || !f0 || !img)
+        return fail(SARF_ERR_INVALID, "sarf_focus: null argument");
+    if (d->K <= 0 || d->Np <= 0 || d->Nx < 2 || d->Ny < 2 || d->Nfft < d->K)
+        return fail(SARF_ERR_INVALID, "sarf_focus: bad sizes K=%d Np=%d Nfft=%d Nx=%d Ny=%d",
+                    d->K, d->Np, d->Nfft, d->Nx, d->Ny);
+    const bool f64 = d->precision == SARF_BP_F64;
+    const size_t csz = f64 ? 16 : 8;
+    const size_t ph_elems = (size_t)d->K * d->Np, prof_elems = (size_t)d->Nfft * d->Np;
+    const size_t img_elems = (size_t)d->Nx * d->Ny;
+    void *dph, *dprof, *dimg, *dgeo;
+    SARF_TRY(scratch_get(ctx, "focus.ph", ph_elems * csz, &dph));
+    SARF_TRY(scratch_get(ctx, "focus.prof", prof_elems * csz, &dprof));
+    SARF_TRY(scratch_get(ctx, "focus.img", img_elems * csz, &dimg));
+    SARF_TRY(scratch_get(ctx, "focus.geo", sizeof(double) * 5 * d->Np, &dgeo));
+    double *dant = (double *)dgeo, *dr0 = dant + 3 * d->Np, *df0 = dr0 + d->Np;
+    hipStream_t s = nullptr;
+    if (f64) {
+        std::vector<double> tmp(2 * ph_elems);
+        for (size_t i = 0; i < 2 * ph_elems; i++) tmp[i] = ph[i];
+        SARF_HIP(hipMemcpy(dph, tmp.data(), ph_elems * csz, hipMemcpyHostToDevice));
+    } else {
+        SARF_HIP(hipMemcpy(dph, ph, ph_elems * csz, hipMemcpyHostToDevice));
+    }
+    SARF_HIP(hipMemcpy(dant, ant, sizeof(double) * 3 * d->Np, hipMemcpyHostToDevice));
+    SARF_HIP(hipMemcpy(dr0, r0, sizeof(double) * d->Np, hipMemcpyHostToDevice));
+    SARF_HIP(hipMemcpy(df0, f0, sizeof(double) * d->Np, hipMemcpyHostToDevice));
+    sarf_rc_desc rc{d->K, d->Np, d->Nfft, f64 ? SARF_RC_F64 : 0};
+    SARF_TRY(launch_range_compress(ctx, &rc, dph, nullptr, nullptr, nullptr, dprof, s));
+    sarf_bp_desc bp{};
+    bp.Nx = d->Nx; bp.Ny = d->Ny; bp.Nfft = d->Nfft; bp.Np = d->Np;
+    bp.x0 = d->x0; bp.y0 = d->y0; bp.Wx = d->Wx; bp.Wy = d->Wy; bp.maxWr = d->maxWr;
+    bp.precision = d->precision;
+    SARF_TRY(launch_backproject(ctx, &bp, dprof, dant, dr0, df0, dimg, s));
+    SARF_HIP(hipStreamSynchronize(s));
+    if (f64) {
+        std::vector<double> tmp(2 * img_elems);
+        SARF_HIP(hipMemcpy(tmp.data(), dimg, img_elems * csz, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < 2 * img_elems; i++) img[i] = (float)tmp[i];
+    } else {
+        SARF_HIP(hipMemcpy(img, dimg, img_elems * csz, hipMemcpyDeviceToHost));
+    }
+    return SARF_OK;
+}
+
+}  // extern "C"

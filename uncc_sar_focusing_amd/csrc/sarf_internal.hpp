@@ -1,0 +1,92 @@
+// sarf_internal.hpp -- shared host/device declarations of libsarf (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "sarf.h"
+
+namespace sarf {
+
+constexpr double kClight = 299792458.0;   // uncc_sar_focusing.hpp:34
+constexpr double kPi = 3.141592653589793; // uncc_sar_focusing.hpp:36
+
+// Records a printf-style message as the calling thread's last error and
+// returns `code`, so call sites read `return fail(SARF_ERR_INVALID, ...)`.
+int fail(int code, const char *fmt, ...);
+void clear_error();
+
+#define SARF_HIP(expr)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ::sarf::fail(SARF_ERR_HIP, "%s: %s (%s:%d)", #expr,                   \
+                                hipGetErrorString(e_), __FILE__, __LINE__);              \
+    } while (0)
+
+#define SARF_TRY(expr)                    \
+    do {                                  \
+        int rc_ = (expr);                 \
+        if (rc_ != SARF_OK) return rc_;   \
+    } while (0)
+
+// Device scratch that grows on demand and is reused between calls.
+struct Scratch {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace sarf
+
+struct sarf_ctx {
+    int device = 0;
+    int cu_count = 0;
+    std::mutex mu;
+    // FFT twiddle tables e^{+j 2 pi i / N}, keyed by log2(N)
+    std::map<int, float2 *> tw32;
+    std::map<int, double2 *> tw64;
+    // named scratch buffers (partial images, reductions, staging)
+    std::map<std::string, sarf::Scratch> scratch;
+};
+
+namespace sarf {
+
+// Makes ctx->device current.
+int use_device(sarf_ctx *ctx);
+// Returns a device buffer of at least `bytes` bytes under `name`.
+int scratch_get(sarf_ctx *ctx, const char *name, size_t bytes, void **out);
+// Twiddle table for N = 2^log2n (created on first use, synchronously).
+int twiddles_f32(sarf_ctx *ctx, int log2n, const float2 **out);
+int twiddles_f64(sarf_ctx *ctx, int log2n, const double2 **out);
+
+int ilog2_exact(long n);  // log2 if n is a power of two, else -1
+
+// Internal launchers shared between translation units.
+int launch_range_compress(sarf_ctx *ctx, const sarf_rc_desc *d, const void *ph,
+                          const float *taper, const double *f0, const double *r0,
+                          void *prof, hipStream_t stream);
+int launch_backproject(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof,
+                       const double *ant, const double *r0, const double *f0, void *img,
+                       hipStream_t stream);
+
+}  // namespace sarf
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}

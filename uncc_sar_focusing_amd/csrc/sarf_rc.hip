@@ -1,0 +1,383 @@
+// sarf_rc.hip -- range compression: prof[n] = fftshift(ifft(ph[n], Nfft)) / ...
+//
+// Semantics (reference): per pulse, zero-pad the K frequency samples to Nfft
+// (matlab/bpBasic.m:82), inverse DFT scaled by 1/Nfft
+// (src/cpuBackProjection/cpuBackProjection_fft.hpp:46-48), then
+// fftshift = cshift((N+1)/2) (:81-84); the GPU reference does the same with
+// cuFFT + cufftNormalize_1DBatch + cufftShift_1DBatch
+// (cuda_BackProjectionKernels.cuh:95-135).  Optional taper and phase ramp
+// follow matlab/rangeCompress.m:6-17.
+//
+// Design (gfx950): one workgroup holds whole pulses in LDS and runs a
+// self-sorting Stockham IFFT, radix 16 per pass (16 complex values per
+// lane), so each pulse is read from HBM once (K samples, the zero padding is
+// never loaded) and written once (Nfft bins).  Fused into the passes:
+//   * fftshift, as a (-1)^k sign on the input samples (N even):
+//       X[(m + N/2) mod N] = sum_k x_k (-1)^k e^{+j 2 pi k m / N};
+//   * taper, 1/N and the ramp, on the first load / last store.
+// LDS rows are padded by one element every 16 so the stride-16 scatter of
+// the first pass is bank-conflict free.  N up to 16384 fits (139 KiB of the
+// 160 KiB LDS).  The FP64 variant (validation builds) uses a global-memory
+// radix-4 Stockham since 16384 double2 do not fit in LDS.
+#include <cmath>
+
+#include "sarf_internal.hpp"
+
+namespace sarf {
+namespace {
+
+// cos(2 pi m / 16) for m = 0..15; sin(x) is cos(x - pi/2).
+template <int M>
+struct Tw16 {
+    static constexpr float c() {
+        constexpr float t[16] = {1.0f, 0.92387953251128674f, 0.70710678118654752f,
+                                 0.38268343236508978f, 0.0f, -0.38268343236508978f,
+                                 -0.70710678118654752f, -0.92387953251128674f, -1.0f,
+                                 -0.92387953251128674f, -0.70710678118654752f,
+                                 -0.38268343236508978f, 0.0f, 0.38268343236508978f,
+                                 0.70710678118654752f, 0.92387953251128674f};
+        return t[M & 15];
+    }
+    static constexpr float s() { return Tw16<(M + 12) & 15>::c(); }  // sin x = cos(x - pi/2)
+};
+
+// d * e^{SIGN j 2 pi M / 16}, with the trivial rotations done exactly.
+template <int SIGN, int M>
+__device__ __forceinline__ float2 rot16(float2 d) {
+    constexpr int m = M & 15;
+    if constexpr (m == 0) {
+        return d;
+    } else if constexpr (m == 8) {
+        return make_float2(-d.x, -d.y);
+    } else if constexpr (m == 4) {  // e^{SIGN j pi/2} = SIGN j
+        return SIGN > 0 ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
+    } else if constexpr (m == 12) {
+        return SIGN > 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
+    } else {
+        constexpr float c = Tw16<m>::c();
+        constexpr float s = SIGN * Tw16<m>::s();
+        return make_float2(d.x * c - d.y * s, d.x * s + d.y * c);
+    }
+}
+
+// Radix-2 decimation-in-frequency network on R registers (R = 2,4,8,16),
+// natural-order input, bit-reversed output.
+template <int SIGN, int L, int S, int K>
+struct DifBfly {
+    static __device__ __forceinline__ void run(float2 *v) {
+        if constexpr (K < L / 2) {
+            float2 a = v[S + K], b = v[S + K + L / 2];
+            v[S + K] = cadd(a, b);
+            v[S + K + L / 2] = rot16<SIGN, (K * 16 / L)>(csub(a, b));
+            DifBfly<SIGN, L, S, K + 1>::run(v);
+        }
+    }
+};
+template <int SIGN, int R, int L, int S>
+struct DifStage {
+    static __device__ __forceinline__ void run(float2 *v) {
+        if constexpr (S < R) {
+            DifBfly<SIGN, L, S, 0>::run(v);
+            DifStage<SIGN, R, L, S + L>::run(v);
+        }
+    }
+};
+template <int SIGN, int R, int L>
+struct DifAll {
+    static __device__ __forceinline__ void run(float2 *v) {
+        if constexpr (L >= 2) {
+            DifStage<SIGN, R, L, 0>::run(v);
+            DifAll<SIGN, R, L / 2>::run(v);
+        }
+    }
+};
+template <int R>
+__host__ __device__ constexpr int bitrev(int x) {
+    int r = 0;
+    for (int b = 1; b < R; b <<= 1) {
+        r = (r << 1) | (x & 1);
+        x >>= 1;
+    }
+    return r;
+}
+
+// In-place R-point DFT (sign SIGN) of v[0..R-1], result in natural order.
+template <int SIGN, int R>
+__device__ __forceinline__ void dft(float2 *v) {
+    DifAll<SIGN, R, R>::run(v);
+    float2 t[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) t[i] = v[bitrev<R>(i)];
+#pragma unroll
+    for (int i = 0; i < R; i++) v[i] = t[i];
+}
+
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+
+// w^r for r = 0..15 from w (r = 0 unused); error <= ~4 roundings.
+__device__ __forceinline__ void powers16(float2 w, float2 *p) {
+    p[1] = w;
+    p[2] = cmul(w, w);
+    p[3] = cmul(p[2], w);
+    p[4] = cmul(p[2], p[2]);
+    p[5] = cmul(p[4], w);
+    p[6] = cmul(p[4], p[2]);
+    p[7] = cmul(p[4], p[3]);
+    p[8] = cmul(p[4], p[4]);
+#pragma unroll
+    for (int r = 9; r < 16; r++) p[r] = cmul(p[8], p[r - 8]);
+}
+
+constexpr int rc_threads(int log2n) {
+    return ((1 << log2n) / 16 >= 256) ? (1 << log2n) / 16 : 256;
+}
+
+template <int LOG2N>
+__global__ __launch_bounds__(rc_threads(LOG2N)) void rc_lds_f32(
+    const float2 *__restrict__ ph, int K, int Np, int flags, const float *__restrict__ taper,
+    const double *__restrict__ f0, const double *__restrict__ r0,
+    const float2 *__restrict__ tw, float2 *__restrict__ prof) {
+    constexpr int N = 1 << LOG2N;
+    constexpr int T = N / 16;                  // lanes per pulse
+    constexpr int P = T >= 256 ? 1 : 256 / T;  // pulses per workgroup
+    constexpr int B = LOG2N % 4;
+    constexpr int R0 = B == 0 ? 16 : (1 << B);  // radix of the first pass
+    constexpr int NB0 = 16 / R0;                // first-pass butterflies per lane
+    constexpr int PADN = N + N / 16;
+    __shared__ float2 lds[P * PADN];
+
+    const int tid = threadIdx.x;
+    const int lp = tid / T, j = tid % T;
+    const int n = blockIdx.x * P + lp;
+    const bool active = n < Np;
+    float2 *L = lds + lp * PADN;
+
+    float2 v[16];
+    // ---- pass 0: global -> LDS, radix R0, Ns = 1 (no twiddles) ----------
+    {
+        const float2 *src = ph + (size_t)(active ? n : 0) * K;
+#pragma unroll
+        for (int b = 0; b < NB0; b++) {
+            const int jb = j + b * T;
+#pragma unroll
+            for (int r = 0; r < R0; r++) {
+                const int k = jb + r * (N / R0);
+                float2 x = make_float2(0.f, 0.f);
+                if (active && k < K) {
+                    x = src[k];
+                    if (flags & SARF_RC_TAPER) {
+                        const float w = taper[k];
+                        x.x *= w;
+                        x.y *= w;
+                    }
+                    if (k & 1) x = make_float2(-x.x, -x.y);  // fftshift fold
+                }
+                v[b * R0 + r] = x;
+            }
+            dft<+1, R0>(v + b * R0);
+#pragma unroll
+            for (int r = 0; r < R0; r++) L[lpad(jb * R0 + r)] = v[b * R0 + r];
+        }
+    }
+    __syncthreads();
+
+    // ---- radix-16 passes --------------------------------------------------
+    constexpr int NPASS16 = (LOG2N - (B == 0 ? 4 : B)) / 4;
+    int Ns = R0;
+#pragma unroll
+    for (int pass = 0; pass < NPASS16; pass++) {
+        const bool last = (pass == NPASS16 - 1);
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r] = L[lpad(j + r * (N / 16))];
+        const int k = j & (Ns - 1);
+        if (k != 0) {
+            float2 p[16];
+            powers16(tw[k * (N / (Ns * 16))], p);
+#pragma unroll
+            for (int r = 1; r < 16; r++) v[r] = cmul(v[r], p[r]);
+        }
+        dft<+1, 16>(v);
+        if (!last) {
+            __syncthreads();  // every lane has read before anyone overwrites
+            const int base = (j - k) * 16 + k;
+#pragma unroll
+            for (int r = 0; r < 16; r++) L[lpad(base + r * Ns)] = v[r];
+            __syncthreads();
+        } else if (active) {
+            // Ns = N/16: output index j + r*N/16, coalesced per r
+            float sr = 1.0f / N, si = 0.0f;
+            if (flags & SARF_RC_RAMP) {
+                double a = -4.0 * kPi * f0[n] * r0[n] / kClight;
+                sr = (float)(cos(a) / N);
+                si = (float)(sin(a) / N);
+            }
+            float2 *dst = prof + (size_t)n * N;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const float2 x = v[r];
+                dst[j + r * (N / 16)] = make_float2(x.x * sr - x.y * si, x.x * si + x.y * sr);
+            }
+        }
+        Ns *= 16;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 path: global-memory Stockham, radix 4 (one radix-2 pass if log2 N is
+// odd).  Pass 0 reads the phase history (zero padding, taper, fftshift
+// fold); the last pass applies 1/N and the ramp.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(256) void rc_pass_f64(
+    const double2 *__restrict__ in, double2 *__restrict__ out, int N, int Ns, int Np,
+    const double2 *__restrict__ tw, int first, int K, int flags,
+    const float *__restrict__ taper, int last, const double *__restrict__ f0,
+    const double *__restrict__ r0) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long per = N / R;
+    const long n = gid / per;
+    const int j = (int)(gid % per);
+    if (n >= Np) return;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int idx = j + r * (N / R);
+        if (first) {
+            double2 x = make_double2(0.0, 0.0);
+            if (idx < K) {
+                x = in[n * K + idx];
+                if (flags & SARF_RC_TAPER) {
+                    x.x *= taper[idx];
+                    x.y *= taper[idx];
+                }
+                if (idx & 1) x = make_double2(-x.x, -x.y);
+            }
+            v[r] = x;
+        } else {
+            v[r] = in[n * N + idx];
+        }
+    }
+    const int k = j % Ns;
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+        const long ti = (long)r * k * (N / (Ns * R));
+        v[r] = cmul(v[r], tw[ti]);
+    }
+    // inverse DFT of size R (R = 2 or 4)
+    if constexpr (R == 2) {
+        double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else {
+        double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        double2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+        double2 jb1 = make_double2(-b1.y, b1.x);  // +j * b1
+        v[0] = cadd(a0, b0);
+        v[2] = csub(a0, b0);
+        v[1] = cadd(a1, jb1);
+        v[3] = csub(a1, jb1);
+    }
+    double sr = 1.0, si = 0.0;
+    if (last) {
+        sr = 1.0 / N;
+        if (flags & SARF_RC_RAMP) {
+            const double a = -4.0 * kPi * f0[n] * r0[n] / kClight;
+            sr = cos(a) / N;
+            si = sin(a) / N;
+        }
+    }
+    const int base = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        double2 x = v[r];
+        if (last) x = make_double2(x.x * sr - x.y * si, x.x * si + x.y * sr);
+        out[n * N + base + r * Ns] = x;
+    }
+}
+
+template <int LOG2N>
+int launch_lds(const sarf_rc_desc *d, const float2 *ph, const float *taper, const double *f0,
+               const double *r0, const float2 *tw, float2 *prof, hipStream_t s) {
+    constexpr int N = 1 << LOG2N;
+    constexpr int T = N / 16;
+    constexpr int P = T >= 256 ? 1 : 256 / T;
+    const int grid = (d->Np + P - 1) / P;
+    hipLaunchKernelGGL(rc_lds_f32<LOG2N>, dim3(grid), dim3(T * P), 0, s, ph, d->K, d->Np,
+                       d->flags, taper, f0, r0, tw, prof);
+    SARF_HIP(hipGetLastError());
+    return SARF_OK;
+}
+
+}  // namespace
+
+int launch_range_compress(sarf_ctx *ctx, const sarf_rc_desc *d, const void *ph,
+                          const float *taper, const double *f0, const double *r0,
+                          void *prof, hipStream_t s) {
+    const int log2n = ilog2_exact(d->Nfft);
+    if (d->K < 1 || d->Np < 1 || d->K > d->Nfft || log2n < 0)
+        return fail(SARF_ERR_INVALID, "range_compress: need 1 <= K <= Nfft, Nfft a power of two "
+                    "(K=%d Np=%d Nfft=%d)", d->K, d->Np, d->Nfft);
+    if ((d->flags & SARF_RC_TAPER) && !taper)
+        return fail(SARF_ERR_INVALID, "range_compress: SARF_RC_TAPER without taper");
+    if ((d->flags & SARF_RC_RAMP) && (!f0 || !r0))
+        return fail(SARF_ERR_INVALID, "range_compress: SARF_RC_RAMP without f0/r0");
+    if (d->flags & SARF_RC_F64) {
+        if (log2n < 1 || log2n > 24)
+            return fail(SARF_ERR_UNSUPPORTED, "range_compress f64: Nfft=%d", d->Nfft);
+        const double2 *tw = nullptr;
+        SARF_TRY(twiddles_f64(ctx, log2n, &tw));
+        const size_t bytes = sizeof(double2) * (size_t)d->Nfft * d->Np;
+        void *tmp = nullptr;
+        SARF_TRY(scratch_get(ctx, "rc64.tmp", bytes, &tmp));
+        // passes: radix-2 first when log2n is odd, then radix-4
+        int npass = (log2n & 1) + log2n / 2;
+        // ping-pong so that the last pass lands in prof
+        double2 *bufs[2] = {(double2 *)prof, (double2 *)tmp};
+        int cur = (npass % 2 == 1) ? 0 : 1;  // output buffer index of pass 0
+        const double2 *in = (const double2 *)ph;
+        int Ns = 1, pass = 0;
+        if (log2n & 1) {
+            const long threads = (long)d->Np * (d->Nfft / 2);
+            hipLaunchKernelGGL(rc_pass_f64<2>, dim3((threads + 255) / 256), dim3(256), 0, s, in,
+                               bufs[cur], d->Nfft, Ns, d->Np, tw, 1, d->K, d->flags, taper,
+                               npass == 1, f0, r0);
+            SARF_HIP(hipGetLastError());
+            in = bufs[cur];
+            cur ^= 1;
+            Ns *= 2;
+            pass++;
+        }
+        for (; pass < npass; pass++) {
+            const long threads = (long)d->Np * (d->Nfft / 4);
+            hipLaunchKernelGGL(rc_pass_f64<4>, dim3((threads + 255) / 256), dim3(256), 0, s, in,
+                               bufs[cur], d->Nfft, Ns, d->Np, tw, pass == 0, d->K, d->flags,
+                               taper, pass == npass - 1, f0, r0);
+            SARF_HIP(hipGetLastError());
+            in = bufs[cur];
+            cur ^= 1;
+            Ns *= 4;
+        }
+        return SARF_OK;
+    }
+    if (log2n < 5 || log2n > 14)
+        return fail(SARF_ERR_UNSUPPORTED, "range_compress: Nfft=%d outside [32,16384]", d->Nfft);
+    const float2 *tw = nullptr;
+    SARF_TRY(twiddles_f32(ctx, log2n, &tw));
+    const float2 *x = (const float2 *)ph;
+    float2 *y = (float2 *)prof;
+    switch (log2n) {
+        case 5: return launch_lds<5>(d, x, taper, f0, r0, tw, y, s);
+        case 6: return launch_lds<6>(d, x, taper, f0, r0, tw, y, s);
+        case 7: return launch_lds<7>(d, x, taper, f0, r0, tw, y, s);
+        case 8: return launch_lds<8>(d, x, taper, f0, r0, tw, y, s);
+        case 9: return launch_lds<9>(d, x, taper, f0, r0, tw, y, s);
+        case 10: return launch_lds<10>(d, x, taper, f0, r0, tw, y, s);
+        case 11: return launch_lds<11>(d, x, taper, f0, r0, tw, y, s);
+        case 12: return launch_lds<12>(d, x, taper, f0, r0, tw, y, s);
+        case 13: return launch_lds<13>(d, x, taper, f0, r0, tw, y, s);
+        case 14: return launch_lds<14>(d, x, taper, f0, r0, tw, y, s);
+    }
+    return fail(SARF_ERR_UNSUPPORTED, "range_compress: Nfft=%d", d->Nfft);
+}
+
+}  // namespace sarf
