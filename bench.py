@@ -68,6 +68,8 @@ def make_inputs(ctx, cfg, torch, dev):
 def cpu_baseline(cfg, geom, grid, ph_host, target_s=12.0):
     """Oracle FP32-faithful RC + BP (reference loop order and float-isms) on
     the first P pulses of the workload, full image, all allowed threads."""
+    import oracle
+
     threads = min(16, os.cpu_count() or 1)
     oracle.set_threads(threads)
 
