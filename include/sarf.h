@@ -112,8 +112,12 @@ typedef struct {
     double maxWr;   /* unambiguous range c/(2 mean deltaF) = max_Wy_m */
     int precision;  /* SARF_BP_MIXED | SARF_BP_F64 */
     int flags;      /* SARF_BP_ACCUMULATE */
-    int tile;       /* 0 = autotuned; else a SARF tile id (sarf_autotune) */
-    int pulse_chunk;/* 0 = autotuned; else 8, 16 or 32 */
+    int tile;       /* 0 = default for the pixel spacing; else a tile id 1..7 */
+    int pulse_split;/* 0 = automatic; n >= 1: pulses in n contiguous groups, each
+                       forming a partial image, summed in group order */
+    double los_x, los_y; /* optional hint: horizontal components of the unit line
+                            of sight from the image centre to the aperture centre;
+                            (0,0) = unknown.  Only steers the default tile choice. */
 } sarf_bp_desc;
 
 int sarf_backproject(sarf_ctx *ctx, const sarf_bp_desc *desc,
@@ -123,6 +127,39 @@ int sarf_backproject(sarf_ctx *ctx, const sarf_bp_desc *desc,
                      const double *f0,   /* device, Np (start frequency of each pulse) */
                      void *img,          /* device, complex Ny x Nx (float2; double2 for F64) */
                      void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Image entropy E = -sum p ln p, p = |I|^2 / sum |I|^2 (FP64 sums).         */
+/* out (device, 3 doubles) = {E, S = sum |I|^2, T = sum |I|^2 ln |I|^2}.     */
+/* The focus metric of every autofocus path here; it replaces the           */
+/* reference's incompatible metrics (8-bit dB column entropy,               */
+/* src/autofocus/gridSearchErrorFunctions.cuh:341-447; DFT energy,          */
+/* gridSearchErrorFunctions_dft.cuh:112-203).                               */
+/* ------------------------------------------------------------------------ */
+int sarf_image_entropy(sarf_ctx *ctx, const void *img, /* device complex, npix */
+                       long npix, int f64 /* 0: float2, 1: double2 */,
+                       double *out, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Grid-search objective for many candidate trajectories in one persistent  */
+/* launch.  Candidate c moves each phase centre along a unit direction:     */
+/*   a_n(c) = a_n + (sum_k coef[c*nbasis + k] * basis[k*Np + n]) * dir_n     */
+/* (e.g. a line-of-sight error a t^2 + b t^3 + c t^4), backprojects all     */
+/* pulses (mixed precision, bp desc as for sarf_backproject) and writes the  */
+/* entropy of candidate c's image to entropy[c]; images are not stored.     */
+/* Replaces kernelWrapper (src/autofocus/gridSearchErrorFunctions_dft.cuh:  */
+/* 258-336) driven by cuGridSearch (src/autofocus/autofocus_gpu.cu:         */
+/* 1057-1070).                                                              */
+/* ------------------------------------------------------------------------ */
+int sarf_bp_batch_entropy(sarf_ctx *ctx, const sarf_bp_desc *desc,
+                          const void *prof,     /* device float2 Nfft x Np */
+                          const double *ant, const double *r0, const double *f0,
+                          int ncand, int nbasis,
+                          const double *coef,   /* device, ncand x nbasis */
+                          const double *basis,  /* device, nbasis x Np */
+                          const double *dir,    /* device, Np x 3 unit vectors */
+                          double *entropy,      /* device, ncand */
+                          void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Whole focusing call on host buffers: H2D, range compression,             */

@@ -103,7 +103,7 @@ def test_bp_c0_f64_matches_oracle(ctx):
     assert oracle.rel_l2(img, ref) < 1e-10
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7])
 def test_bp_every_tile_shape(ctx, tile):
     geom, tg, ph = small_inputs()
     grid = ImageGrid(70, 45, 0.4, -0.3, 7.0, 5.5)  # ragged vs every tile
@@ -200,3 +200,22 @@ def test_synth_matches_oracle(ctx):
                                  to_dev(tg.amp.astype(np.complex64))).cpu().numpy()
     ref = oracle.synth_ph_f64(geom, tg)
     assert oracle.rel_l2(ph, ref) < 2e-5
+
+
+@pytest.mark.parametrize("split", [1, 3, 8, 64])
+def test_bp_pulse_split_partials_match_oracle(ctx, split):
+    """Pulse groups forming partial images summed in order (the path used
+    when an image has too few tiles to fill the GPU)."""
+    geom, tg, ph = small_inputs(Np=100, K=64)
+    grid = ImageGrid.square(40, 0.15)
+    ant, r0, f0 = geom_dev(geom)
+    prof = ctx.range_compress(to_dev(ph), 512)
+    img = ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr, pulse_split=split).cpu().numpy()
+    ref = oracle.bp_f64(oracle.range_compress_f64(ph, 512), geom.ant, geom.r0, geom.f0, grid,
+                        geom.maxWr)
+    assert oracle.rel_l2(img, ref) <= 1e-4
+    # accumulate on top of an existing image
+    base = to_dev(np.full((grid.Nx, grid.Ny), 1 + 2j, np.complex64))
+    ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr, pulse_split=split, out=base,
+                    accumulate=True)
+    assert oracle.rel_l2(base.cpu().numpy() - (1 + 2j), ref) <= 1e-4

@@ -43,7 +43,8 @@ class BpDesc(C.Structure):
         ("Nx", C.c_int), ("Ny", C.c_int), ("Nfft", C.c_int), ("Np", C.c_int),
         ("x0", C.c_double), ("y0", C.c_double), ("Wx", C.c_double), ("Wy", C.c_double),
         ("maxWr", C.c_double),
-        ("precision", C.c_int), ("flags", C.c_int), ("tile", C.c_int), ("pulse_chunk", C.c_int),
+        ("precision", C.c_int), ("flags", C.c_int), ("tile", C.c_int), ("pulse_split", C.c_int),
+        ("los_x", C.c_double), ("los_y", C.c_double),
     ]
 
 
@@ -69,6 +70,9 @@ SIGNATURES = {
     "sarf_backproject": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _vp, _vp]),
     "sarf_focus": (_i, [_vp, C.POINTER(FocusDesc), _vp, _vp, _vp, _vp, _vp]),
     "sarf_synth_phase_history": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp]),
+    "sarf_image_entropy": (_i, [_vp, _vp, C.c_long, _i, _vp, _vp]),
+    "sarf_bp_batch_entropy": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _i, _i, _vp, _vp,
+                                   _vp, _vp, _vp]),
 }
 
 

@@ -94,7 +94,11 @@ class Context:
     def backproject(self, prof: torch.Tensor, ant: torch.Tensor, r0: torch.Tensor,
                     f0: torch.Tensor, grid: ImageGrid, maxWr: float, *,
                     precision: str = "mixed", out: torch.Tensor | None = None,
-                    accumulate: bool = False, tile: int = 0) -> torch.Tensor:
+                    accumulate: bool = False, tile: int = 0,
+                    pulse_split: int = 0, los: tuple[float, float] | None = None) -> torch.Tensor:
+        """Backproject profiles onto `grid`.  `los` optionally gives the
+        horizontal unit line of sight (x, y) from the image centre to the
+        aperture centre; it only steers the default tile shape."""
         Np, nfft = prof.shape
         f64 = precision == "f64"
         cdt = torch.complex128 if f64 else torch.complex64
@@ -108,9 +112,51 @@ class Context:
         _need(out, cdt, (grid.Nx, grid.Ny), "out")
         d = BpDesc(grid.Nx, grid.Ny, nfft, Np, grid.x0, grid.y0, grid.Wx, grid.Wy, maxWr,
                    _lib.SARF_BP_F64 if f64 else _lib.SARF_BP_MIXED,
-                   _lib.SARF_BP_ACCUMULATE if accumulate else 0, tile, 0)
+                   _lib.SARF_BP_ACCUMULATE if accumulate else 0, tile, pulse_split,
+                   *(los if los is not None else (0.0, 0.0)))
         check(self.lib.sarf_backproject(self._h, C.byref(d), _ptr(prof), _ptr(ant), _ptr(r0),
                                         _ptr(f0), _ptr(out), self._stream()))
+        return out
+
+    # ------------------------------------------------------------- entropy
+    def image_entropy(self, img: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Device tensor [E, S, T] (float64): E = -sum p ln p, p = |I|^2/S."""
+        if img.dtype not in (torch.complex64, torch.complex128) or not img.is_cuda:
+            raise ValueError("img must be a complex CUDA tensor")
+        if not img.is_contiguous():
+            raise ValueError("img must be contiguous")
+        if out is None:
+            out = torch.empty(3, dtype=torch.float64, device=img.device)
+        check(self.lib.sarf_image_entropy(self._h, _ptr(img), img.numel(),
+                                          1 if img.dtype == torch.complex128 else 0, _ptr(out),
+                                          self._stream()))
+        return out
+
+    def batch_entropy(self, prof: torch.Tensor, ant: torch.Tensor, r0: torch.Tensor,
+                      f0: torch.Tensor, grid: ImageGrid, maxWr: float, coef: torch.Tensor,
+                      basis: torch.Tensor, direction: torch.Tensor, *, tile: int = 0,
+                      los: tuple[float, float] | None = None,
+                      out: torch.Tensor | None = None) -> torch.Tensor:
+        """Entropy of the image of every candidate trajectory
+        a_n + (coef[c] . basis[:, n]) * direction[n], one persistent launch."""
+        Np, nfft = prof.shape
+        ncand, nb = coef.shape
+        _need(prof, torch.complex64, (Np, nfft), "prof")
+        _need(ant, torch.float64, (Np, 3), "ant")
+        _need(r0, torch.float64, (Np,), "r0")
+        _need(f0, torch.float64, (Np,), "f0")
+        _need(coef, torch.float64, (ncand, nb), "coef")
+        _need(basis, torch.float64, (nb, Np), "basis")
+        _need(direction, torch.float64, (Np, 3), "direction")
+        if out is None:
+            out = torch.empty(ncand, dtype=torch.float64, device=prof.device)
+        _need(out, torch.float64, (ncand,), "out")
+        d = BpDesc(grid.Nx, grid.Ny, nfft, Np, grid.x0, grid.y0, grid.Wx, grid.Wy, maxWr,
+                   _lib.SARF_BP_MIXED, 0, tile, 0, *(los if los is not None else (0.0, 0.0)))
+        check(self.lib.sarf_bp_batch_entropy(self._h, C.byref(d), _ptr(prof), _ptr(ant),
+                                             _ptr(r0), _ptr(f0), ncand, nb, _ptr(coef),
+                                             _ptr(basis), _ptr(direction), _ptr(out),
+                                             self._stream()))
         return out
 
     # --------------------------------------------------------------- focus

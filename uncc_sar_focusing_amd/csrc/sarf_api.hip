@@ -206,6 +206,23 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const d
     bp.Nx = d->Nx; bp.Ny = d->Ny; bp.Nfft = d->Nfft; bp.Np = d->Np;
     bp.x0 = d->x0; bp.y0 = d->y0; bp.Wx = d->Wx; bp.Wy = d->Wy; bp.maxWr = d->maxWr;
     bp.precision = d->precision;
+    {
+        // line-of-sight hint from the mean phase centre (tile choice only)
+        double mx = 0, my = 0, mz = 0;
+        for (int n = 0; n < d->Np; n++) {
+            mx += ant[3 * n];
+            my += ant[3 * n + 1];
+            mz += ant[3 * n + 2];
+        }
+        mx = mx / d->Np - d->x0;
+        my = my / d->Np - d->y0;
+        mz /= d->Np;
+        const double nrm = std::sqrt(mx * mx + my * my + mz * mz);
+        if (nrm > 0) {
+            bp.los_x = mx / nrm;
+            bp.los_y = my / nrm;
+        }
+    }
     SARF_TRY(launch_backproject(ctx, &bp, dprof, dant, dr0, df0, dimg, s));
     SARF_HIP(hipStreamSynchronize(s));
     if (f64) {

@@ -1,71 +1,17 @@
-// sarf_bp.hip -- backprojection of range profiles onto the z = 0 pixel grid.
+// sarf_bp.hip -- image-forming backprojection kernels and their launcher.
 //
-// Reference semantics (src/cpuBackProjection/cpuBackProjection.hpp:24-93):
-// for every pulse n and pixel p,
-//   dR  = |a_n - p| - R0_n                                   (:70-74)
-//   if r_0 < dR < r_{N-1}, r_i = (i/N - 0.5) maxWr           (:76, :45)
-//     pos = (dR/maxWr + 0.5) N, left bin i = floor(pos)      (:83, :28-29)
-//     v   = s_i + (pos - i)(s_{i+1} - s_i)                   (:38-40)
-//     img[x*Ny + y] += v * exp(j 4 pi f0_n dR / c)           (:78-79, :87)
-// The interpolation rule is the CPU reference's (first bin with r > dR),
-// not the GPU reference's roundf (cuda_BackProjectionKernels.cuh:68-69).
-//
-// Mixed-precision kernel (SARF_BP_MIXED) -- design for gfx950:
-//   * A workgroup (256 lanes = 4 wave64) owns a TX x TY pixel tile; each lane
-//     owns PX pixels of one image row (same y), lanes run along y so image
-//     stores are contiguous (y fastest, column-major Ny x Nx).
-//   * Pulses are processed in chunks.  For each (tile, pulse) lanes compute
-//     in FP64: u = a_n - p_c (p_c = tile centre), R_c = |u|, dR_c = R_c - R0,
-//     the tile's profile window, and the phase of the centre reduced to
-//     revolutions, psi_c = frac(2 f0 dR_c / c).  Only these reductions need
-//     more than FP32: at R ~ 12.5 km an FP32 range has a 1 mm ulp (0.4 rad
-//     at 9.6 GHz).
-//   * Per pixel, with q = p - p_c (|q| of a few metres, exact enough in
-//     FP32) and the unit vector uh = u/R_c:
-//         s = uh.q,  t = |q|^2 - s^2,
-//         dR - dR_c = -s + t / (2 (R_c - s)) + O(t^2/R_c^3)
-//     which needs no square root; 1/(2(R_c - s)) ~ g0 + g1 s.
-//   * The window of profile bins each pulse needs for the tile (found from
-//     the tile corners) is staged through LDS as {s_i, s_{i+1} - s_i} pairs,
-//     so the lerp is one 16-byte LDS read and two FMAs.
-//   * Phase in revolutions: psi = frac(psi_c + (2 f0/c) (dR - dR_c)), then
-//     v_sin_f32 / v_cos_f32, which take revolutions.
-//   FP32 for interpolation, rotation and accumulation; no MFMA (this is not
-//   a contraction).
-//
-// FP64 kernel (SARF_BP_F64): one lane per pixel, every quantity in double,
-// the reference's search rule and FP64 sin/cos: the "all-FP64 build" the
-// mixed kernel is measured against at full size.
+// bp_mixed (SARF_BP_MIXED): the tile engine of sarf_bp_core.hpp over plain
+// antenna positions, writing the image (or a per-pulse-group partial).
+// bp_f64 (SARF_BP_F64): one lane per pixel, every quantity in double, the
+// reference's search rule (cpuBackProjection.hpp:28-29) and FP64 sin/cos:
+// the "all-FP64 build" the mixed kernel is measured against at full size.
+#include <algorithm>
 #include <cmath>
 
-#include "sarf_internal.hpp"
+#include "sarf_bp_core.hpp"
 
 namespace sarf {
 namespace {
-
-constexpr int kBpThreads = 256;
-constexpr int kBpChunk = 32;    // max pulses staged per chunk
-constexpr int kBpCap = 2048;    // LDS window capacity, in bins (16 B each)
-
-struct BpArgs {
-    int Nx, Ny, Nfft, Np;
-    int tiles_x, tiles_y;
-    int accumulate;
-    double left, bottom, dx, dy;  // pixel (i,j) = (left + i dx, bottom + j dy)
-    double maxWr;
-    float invbin;                 // Nfft / maxWr (bins per metre)
-};
-
-// XCD-aware tile order: workgroup b runs on XCD b % 8 (round-robin
-// dispatch); give each XCD a contiguous run of tiles.  Tiles are numbered
-// x fastest, so a run is a band of image rows at similar range and its
-// profile windows overlap in that XCD's L2.  Bijective on [0, ntiles).
-__device__ __forceinline__ int swizzle_tile(int b, int ntiles) {
-    const int full = ntiles & ~7;
-    if (b >= full) return b;
-    const int per = full >> 3;
-    return (b & 7) * per + (b >> 3);
-}
 
 template <int TY, int PX>
 __global__ __launch_bounds__(kBpThreads) void bp_mixed(BpArgs a, const float2 *__restrict__ prof,
@@ -73,163 +19,48 @@ __global__ __launch_bounds__(kBpThreads) void bp_mixed(BpArgs a, const float2 *_
                                                        const double *__restrict__ r0,
                                                        const double *__restrict__ f0,
                                                        float2 *__restrict__ img) {
-    constexpr int GX = kBpThreads / TY;  // lane groups along x
-    constexpr int TX = GX * PX;
-    __shared__ float4 win[kBpCap];
-    __shared__ float4 pA[kBpChunk];  // uh_x, uh_y, g0, g1
-    __shared__ float4 pB[kBpChunk];  // pos_c (window-local), psi_c, kappa, gate flag
-    __shared__ int4 pI[kBpChunk];    // start bin, LDS offset (bins), W, interior
-    __shared__ int sNc;
+    __shared__ BpLds L;
+    // blockIdx.x = split * ntiles + tile: every split of a tile covers a
+    // contiguous pulse range and writes its own partial image
+    const int ntiles = a.tiles_x * a.tiles_y;
+    const int split = blockIdx.x / ntiles;
+    const int t = swizzle_tile(blockIdx.x - split * ntiles, ntiles);
+    const int per_split = (a.Np + a.splits - 1) / a.splits;
+    const int nbeg = min(a.Np, split * per_split);
+    const int nend = min(a.Np, nbeg + per_split);
+    img += (size_t)split * a.split_stride;
 
-    const int t = swizzle_tile(blockIdx.x, a.tiles_x * a.tiles_y);
-    const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
-    const int ix0 = tx * TX, iy0 = ty * TY;
-    const double cxi = ix0 + 0.5 * (TX - 1), cyi = iy0 + 0.5 * (TY - 1);
-    const double pcx = a.left + cxi * a.dx, pcy = a.bottom + cyi * a.dy;
-    const double hx = 0.5 * (TX - 1) * a.dx, hy = 0.5 * (TY - 1) * a.dy;
-
-    const int tid = threadIdx.x;
-    const int ly = tid % TY, gx = tid / TY;
-    const float qy = (float)((ly - 0.5 * (TY - 1)) * a.dy);
-    float qx[PX], q2[PX], accr[PX], acci[PX];
+    BpTile<TY, PX> T;
+    T.init(a, t);
+    T.run(a, L, prof, PlainPos{ant}, r0, f0, nbeg, nend);
 #pragma unroll
     for (int j = 0; j < PX; j++) {
-        qx[j] = (float)((gx * PX + j - 0.5 * (TX - 1)) * a.dx);
-        q2[j] = qx[j] * qx[j] + qy * qy;
-        accr[j] = 0.f;
-        acci[j] = 0.f;
-    }
-    const float invbin = a.invbin;
-
-    for (int n0 = 0; n0 < a.Np;) {
-        const int cmax = min(kBpChunk, a.Np - n0);
-        __syncthreads();  // previous chunk finished with LDS
-        if (tid < cmax) {
-            const int n = n0 + tid;
-            const double ux = ant[3 * n] - pcx, uy = ant[3 * n + 1] - pcy, uz = ant[3 * n + 2];
-            const double Rc = sqrt(ux * ux + uy * uy + uz * uz);
-            const double iR = 1.0 / Rc;
-            const double dRc = Rc - r0[n];
-            const double posg = (dRc / a.maxWr + 0.5) * a.Nfft;
-            // |dR - dR_c| over the tile <= |uh_x| hx + |uh_y| hy + |q|^2/(2 R_c)
-            const double ext = fabs(ux * iR) * hx + fabs(uy * iR) * hy + (hx * hx + hy * hy) * 0.5 * iR;
-            const double span = ext * a.Nfft / a.maxWr;
-            const long start = (long)floor(posg - span) - 1;
-            // Clamped so a pulse always fits; the host checks the worst case so
-            // the clamp never binds for valid geometry.
-            const int W = min((int)((long)ceil(posg + span) - start) + 2, kBpCap);
-            const double kap = 2.0 * f0[n] / kClight;  // revolutions per metre
-            double psi = kap * dRc;
-            psi -= rint(psi);
-            pA[tid] = make_float4((float)(ux * iR), (float)(uy * iR), (float)(0.5 * iR),
-                                  (float)(0.5 * iR * iR));
-            pB[tid] = make_float4((float)(posg - (double)start), (float)psi, (float)kap, 0.f);
-            const int interior = (start >= 0 && start + W <= a.Nfft) ? 1 : 0;
-            pI[tid] = make_int4((int)start, 0, W, interior);
-        }
-        __syncthreads();
-        if (tid == 0) {
-            // pack windows back to back while they fit
-            int off = 0, c = 0;
-            for (; c < cmax; c++) {
-                const int W = pI[c].z;
-                if (off + W > kBpCap) break;
-                pI[c].y = off;
-                off += W;
-            }
-            sNc = c;
-        }
-        __syncthreads();
-        const int nc = sNc;  // >= 1: a single window always fits (host-checked)
-
-        // stage {s_b, s_{b+1} - s_b}: wave w takes pulses w, w+4, ...
-        {
-            const int wave = tid >> 6, lane = tid & 63;
-            for (int c = wave; c < nc; c += kBpThreads / 64) {
-                const int4 I = pI[c];
-                const float2 *src = prof + (size_t)(n0 + c) * a.Nfft;
-                for (int b = lane; b < I.z; b += 64) {
-                    const long g = (long)I.x + b;
-                    float2 s0 = make_float2(0.f, 0.f), s1 = make_float2(0.f, 0.f);
-                    if (g >= 0 && g < a.Nfft) s0 = src[g];
-                    if (g + 1 >= 0 && g + 1 < a.Nfft) s1 = src[g + 1];
-                    win[I.y + b] = make_float4(s0.x, s0.y, s1.x - s0.x, s1.y - s0.y);
-                }
-            }
-        }
-        __syncthreads();
-
-        for (int c = 0; c < nc; c++) {
-            const float4 A = pA[c];
-            const float4 B = pB[c];
-            const int4 I = pI[c];
-            const float sy = A.y * qy;
-            const int wmax = I.z - 2;
-            if (I.w) {
-#pragma unroll
-                for (int j = 0; j < PX; j++) {
-                    const float s = fmaf(A.x, qx[j], sy);
-                    const float tq = fmaf(-s, s, q2[j]);
-                    const float g = fmaf(s, A.w, A.z);
-                    const float dl = fmaf(tq, g, -s);  // dR - dR_c
-                    const float pos = fmaf(dl, invbin, B.x);
-                    const float fr = __builtin_amdgcn_fractf(pos);
-                    const int i = min(max((int)pos, 0), wmax);
-                    const float4 w = win[I.y + i];
-                    const float vr = fmaf(fr, w.z, w.x);
-                    const float vi = fmaf(fr, w.w, w.y);
-                    float ph = fmaf(dl, B.z, B.y);
-                    ph = __builtin_amdgcn_fractf(ph);
-                    const float sn = __builtin_amdgcn_sinf(ph);
-                    const float cs = __builtin_amdgcn_cosf(ph);
-                    accr[j] = fmaf(vr, cs, fmaf(-vi, sn, accr[j]));
-                    acci[j] = fmaf(vr, sn, fmaf(vi, cs, acci[j]));
-                }
+        int x, y;
+        if (T.pixel(j, x, y, a)) {
+            float2 *o = img + (size_t)x * a.Ny + y;
+            if (a.accumulate) {
+                const float2 old = *o;
+                *o = make_float2(old.x + T.accr[j], old.y + T.acci[j]);
             } else {
-                // swath gate r_0 < dR < r_{N-1}  <=>  0 < pos_global < N - 1
-                const float glo = (float)(-I.x);
-                const float ghi = (float)(a.Nfft - 1 - I.x);
-#pragma unroll
-                for (int j = 0; j < PX; j++) {
-                    const float s = fmaf(A.x, qx[j], sy);
-                    const float tq = fmaf(-s, s, q2[j]);
-                    const float g = fmaf(s, A.w, A.z);
-                    const float dl = fmaf(tq, g, -s);
-                    const float pos = fmaf(dl, invbin, B.x);
-                    const float fr = __builtin_amdgcn_fractf(pos);
-                    const int i = min(max((int)pos, 0), wmax);
-                    const float4 w = win[I.y + i];
-                    const bool ok = pos > glo && pos < ghi;
-                    const float vr = ok ? fmaf(fr, w.z, w.x) : 0.f;
-                    const float vi = ok ? fmaf(fr, w.w, w.y) : 0.f;
-                    float ph = fmaf(dl, B.z, B.y);
-                    ph = __builtin_amdgcn_fractf(ph);
-                    const float sn = __builtin_amdgcn_sinf(ph);
-                    const float cs = __builtin_amdgcn_cosf(ph);
-                    accr[j] = fmaf(vr, cs, fmaf(-vi, sn, accr[j]));
-                    acci[j] = fmaf(vr, sn, fmaf(vi, cs, acci[j]));
-                }
+                *o = make_float2(T.accr[j], T.acci[j]);
             }
         }
-        n0 += nc;
     }
+}
 
-    const int y = iy0 + ly;
-    if (y < a.Ny) {
-#pragma unroll
-        for (int j = 0; j < PX; j++) {
-            const int x = ix0 + gx * PX + j;
-            if (x < a.Nx) {
-                float2 *o = img + (size_t)x * a.Ny + y;
-                if (a.accumulate) {
-                    const float2 old = *o;
-                    *o = make_float2(old.x + accr[j], old.y + acci[j]);
-                } else {
-                    *o = make_float2(accr[j], acci[j]);
-                }
-            }
-        }
+// Sums the per-split partial images in split order (deterministic) into img.
+__global__ __launch_bounds__(256) void bp_reduce_splits(const float2 *__restrict__ part,
+                                                        long stride, int splits, long n,
+                                                        int accumulate, float2 *__restrict__ img) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float2 acc = accumulate ? img[i] : make_float2(0.f, 0.f);
+    for (int s = 0; s < splits; s++) {
+        const float2 v = part[s * stride + i];
+        acc.x += v.x;
+        acc.y += v.y;
     }
+    img[i] = acc;
 }
 
 // All-FP64 reference-precision kernel, one lane per pixel.
@@ -286,6 +117,8 @@ constexpr TileShape kTiles[] = {
     {3, 64, 8, 32},  // 32 x 64
     {4, 32, 8, 64},  // 64 x 32
     {5, 16, 4, 64},  // 64 x 16
+    {6, 64, 16, 64}, // 64 x 64
+    {7, 32, 16, 128},// 128 x 32
 };
 
 template <int TY, int PX>
@@ -299,29 +132,44 @@ void launch_mixed_t(const BpArgs &a, int grid, const float2 *prof, const double 
 
 int bp_tile_count() { return (int)(sizeof(kTiles) / sizeof(kTiles[0])); }
 
-// Default tile: the largest tile whose half-extent stays within ~1.65 m in
-// each direction (keeps the FP32 local offsets well inside the 1e-4
-// rel-L2 budget) and whose worst-case window fits the LDS capacity.
-static int pick_tile(const sarf_bp_desc *d, double dx, double dy) {
+// Default tile: the first tile of a measured speed ranking (MI355X, 2048^2
+// at 0.05 m and 256^2 at 0.1 m) whose line-of-sight-projected half extent
+// |los_x| hx + |los_y| hy -- the largest |dR - dR_c| over the tile, which
+// sets the FP32 rounding of the local range -- stays within 2.6 m (about
+// 3e-5 relative L2 measured).  With no LOS hint each axis counts fully and
+// the bound is 3.2 m.  The tile's worst-case window must fit in LDS.
+int bp_pick_tile(const sarf_bp_desc *d, double dx, double dy) {
+    static const int kRank[] = {4, 5, 1, 3, 7, 2, 6};
     const double invbin = d->Nfft / d->maxWr;
-    int best = -1;
-    long best_px = 0;
-    for (const TileShape &t : kTiles) {
-        const double hx = 0.5 * (t.tx - 1) * dx, hy = 0.5 * (t.ty - 1) * dy;
-        if (hx > 1.65 || hy > 1.65) continue;
-        const double wmax = 2.0 * (hx + hy) * invbin + 8;
-        if (wmax > kBpCap) continue;
-        const long px = (long)t.tx * t.ty;
-        if (px > best_px) {
-            best_px = px;
-            best = t.id;
+    const bool hint = (d->los_x != 0.0 || d->los_y != 0.0);
+    const double lx = hint ? std::fabs(d->los_x) : 1.0, ly = hint ? std::fabs(d->los_y) : 1.0;
+    const double bound = hint ? 2.6 : 3.2;
+    for (int id : kRank) {
+        for (const TileShape &t : kTiles) {
+            if (t.id != id) continue;
+            const double hx = 0.5 * (t.tx - 1) * dx, hy = 0.5 * (t.ty - 1) * dy;
+            if (lx * hx + ly * hy > bound) continue;
+            if (2.0 * (hx + hy) * invbin + 8 > kBpCap) continue;
+            return id;
         }
     }
-    return best < 0 ? 1 : best;
+    return 1;
 }
 
-int launch_backproject(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof, const double *ant,
-                       const double *r0, const double *f0, void *img, hipStream_t s) {
+// Returns the tile id if known (and its TX, TY), else 0.
+int bp_tile_shape(int id, int *tx, int *ty) {
+    for (const TileShape &t : kTiles)
+        if (t.id == id) {
+            *tx = t.tx;
+            *ty = t.ty;
+            return id;
+        }
+    return 0;
+}
+
+// Validates a mixed/F64 descriptor and fills the kernel arguments for tile
+// `tile` (0 = default choice).  Splits are left at 1.
+int bp_fill_args(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, BpArgs *out) {
     (void)ctx;
     if (d->Nx < 2 || d->Ny < 2 || d->Nfft < 2 || d->Np < 1)
         return fail(SARF_ERR_INVALID, "backproject: bad sizes Nx=%d Ny=%d Nfft=%d Np=%d", d->Nx,
@@ -342,7 +190,35 @@ int launch_backproject(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof, c
     a.bottom = d->y0 - d->Wy / 2;
     a.maxWr = d->maxWr;
     a.invbin = (float)(d->Nfft / d->maxWr);
+    a.splits = 1;
+    a.split_stride = 0;
+    if (d->precision == SARF_BP_MIXED) {
+        if (tile <= 0) tile = bp_pick_tile(d, a.dx, a.dy);
+        int tx = 0, ty = 0;
+        if (!bp_tile_shape(tile, &tx, &ty))
+            return fail(SARF_ERR_INVALID, "backproject: unknown tile id %d", tile);
+        const double hx = 0.5 * (tx - 1) * a.dx, hy = 0.5 * (ty - 1) * a.dy;
+        const double wmax = 2.0 * (hx + hy) * (d->Nfft / d->maxWr) + 8;
+        if (wmax > kBpCap)
+            return fail(SARF_ERR_UNSUPPORTED,
+                        "backproject: tile %d spans %.0f range bins (> %d); pixel spacing too "
+                        "coarse for the range bin width", tile, wmax, kBpCap);
+        a.tiles_x = (d->Nx + tx - 1) / tx;
+        a.tiles_y = (d->Ny + ty - 1) / ty;
+    }
+    *out = a;
+    return SARF_OK;
+}
 
+int launch_backproject(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof, const double *ant,
+                       const double *r0, const double *f0, void *img, hipStream_t s) {
+    if (d->precision != SARF_BP_MIXED && d->precision != SARF_BP_F64)
+        return fail(SARF_ERR_INVALID, "backproject: unknown precision %d", d->precision);
+    const int tile = d->precision == SARF_BP_MIXED
+                         ? (d->tile > 0 ? d->tile : 0)
+                         : 0;
+    BpArgs a{};
+    SARF_TRY(bp_fill_args(ctx, d, tile, &a));
     if (d->precision == SARF_BP_F64) {
         const long npix = (long)d->Nx * d->Ny;
         hipLaunchKernelGGL(bp_f64, dim3((npix + 255) / 256), dim3(256), 0, s, a,
@@ -350,35 +226,52 @@ int launch_backproject(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof, c
         SARF_HIP(hipGetLastError());
         return SARF_OK;
     }
-    if (d->precision != SARF_BP_MIXED)
-        return fail(SARF_ERR_INVALID, "backproject: unknown precision %d", d->precision);
-
-    const int tile = d->tile > 0 ? d->tile : pick_tile(d, a.dx, a.dy);
-    const TileShape *ts = nullptr;
-    for (const TileShape &t : kTiles)
-        if (t.id == tile) ts = &t;
-    if (!ts) return fail(SARF_ERR_INVALID, "backproject: unknown tile id %d", tile);
-    {
-        const double hx = 0.5 * (ts->tx - 1) * a.dx, hy = 0.5 * (ts->ty - 1) * a.dy;
-        const double wmax = 2.0 * (hx + hy) * (d->Nfft / d->maxWr) + 8;
-        if (wmax > kBpCap)
-            return fail(SARF_ERR_UNSUPPORTED,
-                        "backproject: tile %d spans %.0f range bins (> %d); pixel spacing too "
-                        "coarse for the range bin width", tile, wmax, kBpCap);
+    int tx = 0, ty = 0;
+    const int tid = bp_tile_shape(tile > 0 ? tile : bp_pick_tile(d, a.dx, a.dy), &tx, &ty);
+    const int ntiles = a.tiles_x * a.tiles_y;
+    // Fill the GPU: with fewer than 4 tiles per CU, split the pulses into
+    // groups that form partial images (summed in order afterwards), keeping
+    // at least 64 pulses per group.
+    int splits = 1;
+    if (d->pulse_split > 0) {
+        splits = d->pulse_split;  // explicit (tests, autotuner)
+    } else {
+        const int want = 4 * (ctx->cu_count > 0 ? ctx->cu_count : 256);
+        while (splits < 32 && ntiles * splits < want && d->Np / (2 * splits) >= 64) splits *= 2;
     }
-    a.tiles_x = (d->Nx + ts->tx - 1) / ts->tx;
-    a.tiles_y = (d->Ny + ts->ty - 1) / ts->ty;
-    const int grid = a.tiles_x * a.tiles_y;
-    const float2 *p = (const float2 *)prof;
+    splits = std::max(1, std::min(splits, d->Np));
+    const long npix = (long)d->Nx * d->Ny;
     float2 *o = (float2 *)img;
-    switch (ts->id) {
+    float2 *part = nullptr;
+    a.splits = splits;
+    a.split_stride = 0;
+    if (splits > 1) {
+        void *buf = nullptr;
+        SARF_TRY(scratch_get(ctx, "bp.partials", sizeof(float2) * npix * splits, &buf));
+        part = (float2 *)buf;
+        o = part;
+        a.split_stride = npix;
+        a.accumulate = 0;
+    }
+    const int grid = ntiles * splits;
+    const float2 *p = (const float2 *)prof;
+    switch (tid) {
         case 1: launch_mixed_t<32, 4>(a, grid, p, ant, r0, f0, o, s); break;
         case 2: launch_mixed_t<64, 4>(a, grid, p, ant, r0, f0, o, s); break;
         case 3: launch_mixed_t<64, 8>(a, grid, p, ant, r0, f0, o, s); break;
         case 4: launch_mixed_t<32, 8>(a, grid, p, ant, r0, f0, o, s); break;
         case 5: launch_mixed_t<16, 4>(a, grid, p, ant, r0, f0, o, s); break;
+        case 6: launch_mixed_t<64, 16>(a, grid, p, ant, r0, f0, o, s); break;
+        case 7: launch_mixed_t<32, 16>(a, grid, p, ant, r0, f0, o, s); break;
+        default: return fail(SARF_ERR_INVALID, "backproject: unknown tile id %d", tid);
     }
     SARF_HIP(hipGetLastError());
+    if (splits > 1) {
+        hipLaunchKernelGGL(bp_reduce_splits, dim3((npix + 255) / 256), dim3(256), 0, s, part,
+                           (long)npix, splits, npix, (d->flags & SARF_BP_ACCUMULATE) ? 1 : 0,
+                           (float2 *)img);
+        SARF_HIP(hipGetLastError());
+    }
     return SARF_OK;
 }
 

@@ -1,0 +1,282 @@
+// sarf_bp_core.hpp -- the backprojection tile engine shared by every BP
+// kernel (image formation, candidate batches, adjoint).
+//
+// Reference semantics (src/cpuBackProjection/cpuBackProjection.hpp:24-93):
+// for every pulse n and pixel p,
+//   dR  = |a_n - p| - R0_n                                   (:70-74)
+//   if r_0 < dR < r_{N-1}, r_i = (i/N - 0.5) maxWr           (:76, :45)
+//     pos = (dR/maxWr + 0.5) N, left bin i = floor(pos)      (:83, :28-29)
+//     v   = s_i + (pos - i)(s_{i+1} - s_i)                   (:38-40)
+//     img[x*Ny + y] += v * exp(j 4 pi f0_n dR / c)           (:78-79, :87)
+// The interpolation rule is the CPU reference's (first bin with r > dR),
+// not the GPU reference's roundf (cuda_BackProjectionKernels.cuh:68-69).
+//
+// Design for gfx950 (one workgroup = 256 lanes = 4 wave64 per pixel tile):
+//   * Lanes run along y (contiguous stores, image is y fastest); each lane
+//     owns PX pixels of one row; the tile is TX x TY.
+//   * Pulses go in chunks of up to kBpChunk.  For each (tile, pulse) one lane
+//     computes in FP64: u = a_n - p_c (p_c = tile centre), R_c = |u|,
+//     dR_c = R_c - R0, the profile window the tile needs (from its corners,
+//     exact for any geometry) and psi_c = frac(2 f0 dR_c / c) in revolutions.
+//     Only these need more than FP32: at R ~ 12.5 km an FP32 range has a
+//     1 mm ulp (0.4 rad at 9.6 GHz).
+//   * Per pixel, FP32, q = p - p_c (a few metres), uh = u / R_c:
+//         s = uh.q,  t = |q|^2 - s^2,
+//         dR - dR_c = -s + t (g0 + g1 s),  g0 = 1/(2 R_c), g1 = 1/(2 R_c^2)
+//     (expansion of (R_c - s) sqrt(1 + t/(R_c - s)^2) - R_c; remainder
+//     t^2/(8 R^3)), so no square root per pixel.
+//   * Profile windows are staged in LDS as {s_i, s_{i+1} - s_i} so the lerp
+//     is one 16-byte LDS read and two FMAs; a chunk's windows are packed.
+//   * Phase psi = psi_c + (2 f0/c)(dR - dR_c) in revolutions goes straight
+//     into v_sin_f32 / v_cos_f32 (they take revolutions and reduce exactly
+//     for |psi| <= 256).
+//   * Interior tiles skip the per-pixel swath gate (uniform per pulse).
+#pragma once
+
+#include "sarf_internal.hpp"
+
+namespace sarf {
+
+constexpr int kBpThreads = 256;
+constexpr int kBpChunk = 32;  // max pulses staged per chunk
+constexpr int kBpCap = 2048;  // LDS window capacity, in bins (16 B each)
+
+struct BpArgs {
+    int Nx, Ny, Nfft, Np;
+    int tiles_x, tiles_y;
+    int accumulate;
+    int splits;         // pulse groups per tile (> 1: partial images, reduced after)
+    long split_stride;  // elements between partial images
+    double left, bottom, dx, dy;  // pixel (i,j) = (left + i dx, bottom + j dy)
+    double maxWr;
+    float invbin;  // Nfft / maxWr (bins per metre)
+};
+
+struct BpLds {
+    float4 win[kBpCap];
+    float4 pA[kBpChunk];  // uh_x, uh_y, g0, g1
+    float4 pB[kBpChunk];  // pos_c (window-local), psi_c, kappa, W - 2
+    int4 pI[kBpChunk];    // start bin, LDS offset (bins), W, interior
+    int nc;
+};
+
+// Antenna positions as given.
+struct PlainPos {
+    const double *ant;
+    __device__ __forceinline__ double3 operator()(int n) const {
+        return make_double3(ant[3 * n], ant[3 * n + 1], ant[3 * n + 2]);
+    }
+};
+
+// Candidate trajectory: a_n + (sum_k coef_k basis_k[n]) dir_n -- e.g. a
+// polynomial line-of-sight error a t^2 + b t^3 + c t^4 along the unit LOS.
+struct LosPolyPos {
+    const double *ant, *dir, *basis, *coef;
+    int nbasis, Np;
+    __device__ __forceinline__ double3 operator()(int n) const {
+        double off = 0.0;
+        for (int k = 0; k < nbasis; k++) off = fma(coef[k], basis[(size_t)k * Np + n], off);
+        return make_double3(fma(off, dir[3 * n], ant[3 * n]), fma(off, dir[3 * n + 1], ant[3 * n + 1]),
+                            fma(off, dir[3 * n + 2], ant[3 * n + 2]));
+    }
+};
+
+// XCD-aware tile order: workgroup b runs on XCD b % 8 (round-robin
+// dispatch); give each XCD a contiguous run of tiles.  Tiles are numbered
+// x fastest, so a run is a band of image rows at similar range whose
+// profile windows overlap in that XCD's L2.  Bijective on [0, ntiles).
+__device__ __forceinline__ int swizzle_tile(int b, int ntiles) {
+    const int full = ntiles & ~7;
+    if (b >= full) return b;
+    const int per = full >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+template <int TY, int PX>
+struct BpTile {
+    static constexpr int GX = kBpThreads / TY;  // lane groups along x
+    static constexpr int TX = GX * PX;
+    int ix0, iy0, ly, gx;
+    double pcx, pcy, hx, hy;
+    float qy;
+    float qx[PX], q2[PX], accr[PX], acci[PX];
+
+    __device__ __forceinline__ void init(const BpArgs &a, int t) {
+        const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
+        ix0 = tx * TX;
+        iy0 = ty * TY;
+        pcx = a.left + (ix0 + 0.5 * (TX - 1)) * a.dx;
+        pcy = a.bottom + (iy0 + 0.5 * (TY - 1)) * a.dy;
+        hx = 0.5 * (TX - 1) * a.dx;
+        hy = 0.5 * (TY - 1) * a.dy;
+        const int tid = threadIdx.x;
+        ly = tid % TY;
+        gx = tid / TY;
+        qy = (float)((ly - 0.5 * (TY - 1)) * a.dy);
+#pragma unroll
+        for (int j = 0; j < PX; j++) {
+            qx[j] = (float)((gx * PX + j - 0.5 * (TX - 1)) * a.dx);
+            q2[j] = qx[j] * qx[j] + qy * qy;
+            accr[j] = 0.f;
+            acci[j] = 0.f;
+        }
+    }
+
+    __device__ __forceinline__ bool pixel(int j, int &x, int &y, const BpArgs &a) const {
+        x = ix0 + gx * PX + j;
+        y = iy0 + ly;
+        return x < a.Nx && y < a.Ny;
+    }
+
+    // Per-(tile,pulse) FP64 set-up for pulses [n0, n0 + cmax): fills L.pA,
+    // L.pB, L.pI, packs windows, sets L.nc; then stages the windows.
+    template <class Pos>
+    __device__ __forceinline__ void stage(const BpArgs &a, BpLds &L, const float2 *__restrict__ prof,
+                                          const Pos &pos, const double *__restrict__ r0,
+                                          const double *__restrict__ f0, int n0, int cmax) {
+        const int tid = threadIdx.x;
+        __syncthreads();  // previous users of L are done
+        if (tid < cmax) {
+            const int n = n0 + tid;
+            const double3 an = pos(n);
+            const double ux = an.x - pcx, uy = an.y - pcy, uz = an.z;
+            const double Rc = sqrt(ux * ux + uy * uy + uz * uz);
+            const double iR = 1.0 / Rc;
+            const double dRc = Rc - r0[n];
+            const double posg = (dRc / a.maxWr + 0.5) * a.Nfft;
+            // |dR - dR_c| over the tile <= |uh_x| hx + |uh_y| hy + |q|^2/(2 R_c)
+            const double ext = fabs(ux * iR) * hx + fabs(uy * iR) * hy + (hx * hx + hy * hy) * 0.5 * iR;
+            const double span = ext * a.Nfft / a.maxWr;
+            const long start = (long)floor(posg - span) - 1;
+            // Clamped so a pulse always fits; the host checks the worst case so
+            // the clamp does not bind for valid geometry.
+            const int W = min((int)((long)ceil(posg + span) - start) + 2, kBpCap);
+            const double kap = 2.0 * f0[n] / kClight;  // revolutions per metre
+            double psi = kap * dRc;
+            psi -= rint(psi);
+            L.pA[tid] = make_float4((float)(ux * iR), (float)(uy * iR), (float)(0.5 * iR),
+                                    (float)(0.5 * iR * iR));
+            L.pB[tid] = make_float4((float)(posg - (double)start), (float)psi, (float)kap,
+                                    (float)(W - 2));
+            const int interior = (start >= 0 && start + W <= a.Nfft) ? 1 : 0;
+            L.pI[tid] = make_int4((int)start, 0, W, interior);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int off = 0, c = 0;
+            for (; c < cmax; c++) {
+                const int W = L.pI[c].z;
+                if (off + W > kBpCap) break;
+                L.pI[c].y = off;
+                off += W;
+            }
+            L.nc = c;
+        }
+        __syncthreads();
+        const int nc = L.nc;
+        const int wave = tid >> 6, lane = tid & 63;
+        for (int c = wave; c < nc; c += kBpThreads / 64) {
+            const int4 I = L.pI[c];
+            const float2 *src = prof + (size_t)(n0 + c) * a.Nfft;
+            for (int b = lane; b < I.z; b += 64) {
+                const long g = (long)I.x + b;
+                float2 s0 = make_float2(0.f, 0.f), s1 = make_float2(0.f, 0.f);
+                if (g >= 0 && g < a.Nfft) s0 = src[g];
+                if (g + 1 >= 0 && g + 1 < a.Nfft) s1 = src[g + 1];
+                L.win[I.y + b] = make_float4(s0.x, s0.y, s1.x - s0.x, s1.y - s0.y);
+            }
+        }
+        __syncthreads();
+    }
+
+    // Accumulate the staged chunk into the lane's pixels.
+    __device__ __forceinline__ void accumulate_chunk(const BpArgs &a, const BpLds &L) {
+        const float invbin = a.invbin;
+        const int nc = L.nc;
+        for (int c = 0; c < nc; c++) {
+            const float4 A = L.pA[c];
+            const float4 B = L.pB[c];
+            const int4 I = L.pI[c];
+            const float sy = A.y * qy;
+            const float wmax = B.w;
+            if (I.w) {
+#pragma unroll
+                for (int j = 0; j < PX; j++) {
+                    const float s = fmaf(A.x, qx[j], sy);
+                    const float tq = fmaf(-s, s, q2[j]);
+                    const float g = fmaf(s, A.w, A.z);
+                    const float dl = fmaf(tq, g, -s);  // dR - dR_c
+                    const float p = fmaf(dl, invbin, B.x);
+                    const float fr = __builtin_amdgcn_fractf(p);
+                    const int i = (int)__builtin_amdgcn_fmed3f(p, 0.f, wmax);
+                    const float4 w = L.win[I.y + i];
+                    const float vr = fmaf(fr, w.z, w.x);
+                    const float vi = fmaf(fr, w.w, w.y);
+                    const float ph = fmaf(dl, B.z, B.y);  // revolutions
+                    const float sn = __builtin_amdgcn_sinf(ph);
+                    const float cs = __builtin_amdgcn_cosf(ph);
+                    accr[j] = fmaf(vr, cs, fmaf(-vi, sn, accr[j]));
+                    acci[j] = fmaf(vr, sn, fmaf(vi, cs, acci[j]));
+                }
+            } else {
+                // swath gate r_0 < dR < r_{N-1}  <=>  0 < pos_global < N - 1
+                const float glo = (float)(-I.x);
+                const float ghi = (float)(a.Nfft - 1 - I.x);
+#pragma unroll
+                for (int j = 0; j < PX; j++) {
+                    const float s = fmaf(A.x, qx[j], sy);
+                    const float tq = fmaf(-s, s, q2[j]);
+                    const float g = fmaf(s, A.w, A.z);
+                    const float dl = fmaf(tq, g, -s);
+                    const float p = fmaf(dl, invbin, B.x);
+                    const float fr = __builtin_amdgcn_fractf(p);
+                    const int i = (int)__builtin_amdgcn_fmed3f(p, 0.f, wmax);
+                    const float4 w = L.win[I.y + i];
+                    const bool ok = p > glo && p < ghi;
+                    const float vr = ok ? fmaf(fr, w.z, w.x) : 0.f;
+                    const float vi = ok ? fmaf(fr, w.w, w.y) : 0.f;
+                    const float ph = fmaf(dl, B.z, B.y);
+                    const float sn = __builtin_amdgcn_sinf(ph);
+                    const float cs = __builtin_amdgcn_cosf(ph);
+                    accr[j] = fmaf(vr, cs, fmaf(-vi, sn, accr[j]));
+                    acci[j] = fmaf(vr, sn, fmaf(vi, cs, acci[j]));
+                }
+            }
+        }
+    }
+
+    // All pulses in [nbeg, nend).
+    template <class Pos>
+    __device__ __forceinline__ void run(const BpArgs &a, BpLds &L, const float2 *__restrict__ prof,
+                                        const Pos &pos, const double *__restrict__ r0,
+                                        const double *__restrict__ f0, int nbeg, int nend) {
+        for (int n0 = nbeg; n0 < nend;) {
+            stage(a, L, prof, pos, r0, f0, n0, min(kBpChunk, nend - n0));
+            accumulate_chunk(a, L);
+            n0 += L.nc;
+        }
+    }
+};
+
+// Wave64 + workgroup sum of two doubles; result valid on thread 0.
+__device__ __forceinline__ double2 block_sum2(double a, double b, double2 *red /* [4] */) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[wave] = make_double2(a, b);
+    __syncthreads();
+    double2 r = make_double2(0.0, 0.0);
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+            r.x += red[w].x;
+            r.y += red[w].y;
+        }
+    }
+    return r;
+}
+
+}  // namespace sarf

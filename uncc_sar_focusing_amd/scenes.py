@@ -63,6 +63,13 @@ class Geometry:
     def Np(self) -> int:
         return self.ant.shape[0]
 
+    def los(self, x0: float = 0.0, y0: float = 0.0) -> tuple[float, float]:
+        """Horizontal components of the unit line of sight from (x0, y0, 0)
+        to the aperture's mean phase centre."""
+        u = self.ant.mean(axis=0) - np.array([x0, y0, 0.0])
+        u = u / np.linalg.norm(u)
+        return float(u[0]), float(u[1])
+
     @property
     def maxWr(self) -> float:
         """Unambiguous range extent c/(2 df) (uncc_sar_focusing.hpp:430)."""

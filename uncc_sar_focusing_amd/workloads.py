@@ -1,0 +1,64 @@
+"""Builders for the benchmark workloads, with phase histories synthesised on
+the GPU (libsarf's point-scatterer synthesis, FP64 phase).
+
+  C1  4096 x 2048 samples, Nfft 16384, 2048^2 at 0.05 m, 100 targets (+ clutter)
+  C2  1024 x 1024, Nfft 4096, 512^2 at 0.05 m, 20 targets + clutter on every
+      4th cell at -20 dB; each pulse carries a line-of-sight error
+      a t^2 + b t^3 + c t^4 with a, b, c ~ U(-0.3, 0.3) m (seeded); focusing
+      uses the nominal trajectory.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import torch
+
+from . import autofocus as af
+from .scenes import CONFIGS, Geometry, ImageGrid, Targets, grid_clutter, uniform
+
+
+@dataclasses.dataclass
+class DeviceScene:
+    cfg: object
+    geom: Geometry          # nominal geometry (what the radar reports)
+    true_geom: Geometry     # geometry the data was synthesised with
+    grid: ImageGrid
+    targets: Targets
+    ph: torch.Tensor        # (Np, K) complex64 on the GPU
+    ant: torch.Tensor
+    r0: torch.Tensor
+    f0: torch.Tensor
+    true_coef: np.ndarray | None = None
+    n_clutter: int = 0
+
+
+def _dev(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def synth(ctx, geom: Geometry, tg: Targets, dev="cuda:0", out=None) -> torch.Tensor:
+    ant, r0, f0 = _dev(geom.ant, dev), _dev(geom.r0, dev), _dev(geom.f0, dev)
+    return ctx.synth_phase_history(ant, r0, f0, geom.df, geom.K, _dev(tg.pos, dev),
+                                   _dev(tg.amp.astype(np.complex64), dev), out=out)
+
+
+def los_error_scene(ctx, name: str = "C2", seed: int | None = None, clutter: bool = True,
+                    dev: str = "cuda:0", coef_range: float = 0.3) -> DeviceScene:
+    cfg = CONFIGS[name]
+    seed = cfg.seed if seed is None else seed
+    geom = cfg.geometry()
+    grid = cfg.grid()
+    tg = cfg.targets()
+    u = uniform(seed, 11, 3)
+    true_coef = (2.0 * u - 1.0) * coef_range
+    true_geom = af.perturbed(geom, true_coef, x0=grid.x0, y0=grid.y0)
+    ph = synth(ctx, true_geom, tg, dev)
+    n_cl = 0
+    if clutter and cfg.clutter_stride:
+        cl = grid_clutter(grid, cfg.clutter_stride, cfg.clutter_db, seed)
+        synth(ctx, true_geom, cl, dev, out=ph)
+        n_cl = cl.pos.shape[0]
+    torch.cuda.synchronize()
+    return DeviceScene(cfg, geom, true_geom, grid, tg, ph, _dev(geom.ant, dev), _dev(geom.r0, dev),
+                       _dev(geom.f0, dev), true_coef=true_coef, n_clutter=n_cl)
