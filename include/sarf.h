@@ -162,6 +162,23 @@ int sarf_bp_batch_entropy(sarf_ctx *ctx, const sarf_bp_desc *desc,
                           void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Adjoint backprojection: grad[n] = dE/da_n (FP64, Np x 3), the gradient    */
+/* of the image entropy with respect to each antenna phase centre (R0 held   */
+/* fixed), given the forward image img = sarf_backproject(...) of the same   */
+/* inputs and ent = sarf_image_entropy(img).  Interpolated profile values    */
+/* are recomputed, nothing per pixel-pulse is stored.  No counterpart in the */
+/* reference; it makes the grid search of src/autofocus/autofocus_gpu.cu     */
+/* a gradient search.                                                       */
+/* ------------------------------------------------------------------------ */
+int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *desc,
+                         const void *prof,    /* device float2 Nfft x Np */
+                         const double *ant, const double *r0, const double *f0,
+                         const void *img,     /* device float2 Ny x Nx (forward image) */
+                         const double *ent,   /* device {E, S, T} of img */
+                         double *grad,        /* device, Np x 3 */
+                         void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Whole focusing call on host buffers: H2D, range compression,             */
 /* backprojection, D2H.  This is what cuda_focus_SAR_image<float,float>     */
 /* (cuda_sar_focusing.hpp:526-723) does; it is the entry the C++ host       */

@@ -94,3 +94,84 @@ def test_grid_search_finds_injected_los_error(ctx):
                                                                           direction),
                                             geom.r0, geom.f0, grid, geom.maxWr)) for c in coefs]
     assert int(np.argmin(ref)) == int(np.argmin(res.entropies))
+
+
+# ---------------------------------------------------------------- adjoint
+def _misfocused(Np=40, K=64, n=32, spacing=0.15, seed=31):
+    geom = circular_aperture(Np, K)
+    tg = point_targets(6, n * spacing * 0.7, seed=seed)
+    ph = oracle.synth_ph_f64(geom, tg).astype(np.complex64)
+    ant = geom.ant.copy()
+    t = np.linspace(-1, 1, Np)
+    ant[:, 0] += 0.004 * t ** 2
+    ant[:, 1] += 0.006 * t ** 3 - 0.003 * t
+    ant[:, 2] += 0.002 * t ** 2
+    return geom, ph, ant, ImageGrid.square(n, spacing)
+
+
+def test_adjoint_matches_fp64_oracle_gradient(ctx):
+    """GPU adjoint (mixed precision) vs the FP64 CPU adjoint of the FP64 image.
+    Tolerance: 2e-3 relative in the 2-norm over all pulses and axes."""
+    geom, ph, ant, grid = _misfocused()
+    nfft = 1024
+    prof_h = oracle.range_compress_f64(ph, nfft)
+    prof = to_dev(prof_h.astype(np.complex64))
+    r0, f0 = to_dev(geom.r0), to_dev(geom.f0)
+    img = ctx.backproject(prof, to_dev(ant), r0, f0, grid, geom.maxWr)
+    ent = ctx.image_entropy(img)
+    g = ctx.entropy_grad(prof, to_dev(ant), r0, f0, grid, geom.maxWr, img, ent).cpu().numpy()
+    img_ref = oracle.bp_f64(prof_h, ant, geom.r0, geom.f0, grid, geom.maxWr)
+    g_ref = oracle.bp_adjoint_f64(prof_h, ant, geom.r0, geom.f0, grid, geom.maxWr, img_ref)
+    rel = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    assert rel < 2e-3, rel
+
+
+def test_adjoint_matches_central_differences_on_32_pulses(ctx):
+    """Central differences (h = 1e-4 m) of the FP64 GPU forward model on 32
+    random pulses x 3 axes vs the adjoint.  Tolerance 2e-3 of max |grad|."""
+    geom, ph, ant, grid = _misfocused(Np=96, K=96, n=40, spacing=0.12)
+    nfft = 1024
+    prof_h = oracle.range_compress_f64(ph, nfft)
+    prof32 = to_dev(prof_h.astype(np.complex64))
+    prof64 = to_dev(prof_h)
+    r0, f0 = to_dev(geom.r0), to_dev(geom.f0)
+    img = ctx.backproject(prof32, to_dev(ant), r0, f0, grid, geom.maxWr)
+    g = ctx.entropy_grad(prof32, to_dev(ant), r0, f0, grid, geom.maxWr, img,
+                         ctx.image_entropy(img)).cpu().numpy()
+
+    def E(a):
+        im = ctx.backproject(prof64, to_dev(a), r0, f0, grid, geom.maxWr, precision="f64")
+        return float(ctx.image_entropy(im).cpu()[0])
+
+    rng = np.random.default_rng(7)
+    pulses = rng.choice(geom.Np, 32, replace=False)
+    h = 1e-4
+    gmax = np.abs(g).max()
+    for n in pulses:
+        for ax in range(3):
+            ap, am = ant.copy(), ant.copy()
+            ap[n, ax] += h
+            am[n, ax] -= h
+            fd = (E(ap) - E(am)) / (2 * h)
+            assert abs(fd - g[n, ax]) <= 2e-3 * gmax, (n, ax, fd, g[n, ax], gmax)
+
+
+def test_autograd_chain_rule_through_los_polynomial(ctx):
+    """torch autograd through ant(c) = ant0 + (c . basis) dir equals the
+    explicit chain rule of the per-pulse gradient."""
+    from uncc_sar_focusing_amd.autograd import BpOperator, entropy
+
+    geom, ph, ant, grid = _misfocused()
+    prof = ctx.range_compress(to_dev(ph), 1024)
+    op = BpOperator(ctx, prof, to_dev(geom.r0), to_dev(geom.f0), grid, geom.maxWr)
+    basis = to_dev(af.poly_basis(geom.Np))
+    direction = to_dev(af.los_directions(geom.ant))
+    ant0 = to_dev(ant)
+    c = torch.tensor([0.01, -0.02, 0.005], dtype=torch.float64, device=DEV, requires_grad=True)
+    e = entropy(ant0 + (c @ basis)[:, None] * direction, op)
+    e.backward()
+    a = (ant0 + (c.detach() @ basis)[:, None] * direction).contiguous()
+    img = op.image(a)
+    g = ctx.entropy_grad(prof, a, op.r0, op.f0, grid, geom.maxWr, img, ctx.image_entropy(img))
+    expect = basis @ (g * direction).sum(dim=1)
+    assert torch.allclose(c.grad, expect, rtol=1e-9, atol=1e-12)

@@ -73,6 +73,7 @@ SIGNATURES = {
     "sarf_image_entropy": (_i, [_vp, _vp, C.c_long, _i, _vp, _vp]),
     "sarf_bp_batch_entropy": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _i, _i, _vp, _vp,
                                    _vp, _vp, _vp]),
+    "sarf_bp_entropy_grad": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 

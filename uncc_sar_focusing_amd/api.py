@@ -159,6 +159,29 @@ class Context:
                                              self._stream()))
         return out
 
+    def entropy_grad(self, prof: torch.Tensor, ant: torch.Tensor, r0: torch.Tensor,
+                     f0: torch.Tensor, grid: ImageGrid, maxWr: float, img: torch.Tensor,
+                     ent: torch.Tensor, *, tile: int = 0, los: tuple[float, float] | None = None,
+                     out: torch.Tensor | None = None) -> torch.Tensor:
+        """dE/d(antenna positions), (Np, 3) float64, given the forward image
+        `img` of the same inputs and `ent` = image_entropy(img)."""
+        Np, nfft = prof.shape
+        _need(prof, torch.complex64, (Np, nfft), "prof")
+        _need(ant, torch.float64, (Np, 3), "ant")
+        _need(r0, torch.float64, (Np,), "r0")
+        _need(f0, torch.float64, (Np,), "f0")
+        _need(img, torch.complex64, (grid.Nx, grid.Ny), "img")
+        _need(ent, torch.float64, (3,), "ent")
+        if out is None:
+            out = torch.empty((Np, 3), dtype=torch.float64, device=prof.device)
+        _need(out, torch.float64, (Np, 3), "out")
+        d = BpDesc(grid.Nx, grid.Ny, nfft, Np, grid.x0, grid.y0, grid.Wx, grid.Wy, maxWr,
+                   _lib.SARF_BP_MIXED, 0, tile, 0, *(los if los is not None else (0.0, 0.0)))
+        check(self.lib.sarf_bp_entropy_grad(self._h, C.byref(d), _ptr(prof), _ptr(ant), _ptr(r0),
+                                            _ptr(f0), _ptr(img), _ptr(ent), _ptr(out),
+                                            self._stream()))
+        return out
+
     # --------------------------------------------------------------- focus
     def focus_host(self, ph, ant, r0, f0, grid: ImageGrid, nfft: int, maxWr: float,
                    precision: str = "mixed"):

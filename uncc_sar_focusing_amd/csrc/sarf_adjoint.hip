@@ -1,0 +1,253 @@
+// sarf_adjoint.hip -- adjoint backprojection: the gradient of the image
+// entropy with respect to every antenna phase centre (no reference
+// counterpart; it turns grid-search autofocus into gradient autofocus).
+//
+//   E = -sum_p q_p ln q_p,  q_p = |I_p|^2 / S,  S = sum |I_p|^2
+//   I_p = sum_n v_n(dR_np) e^{j 2 pi kappa_n dR_np},  dR_np = |a_n - p| - R0_n
+//   dE/da_n = sum_p c_np (a_n - p)/|a_n - p|,
+//   c_np = 2 w_p Re( conj(I_p) (v'_n + j 2 pi kappa_n v_n) e^{j 2 pi kappa_n dR} )
+//   w_p  = dE/d|I_p|^2 = -(ln q_p + E) / S
+// with v the linear interpolant of the range profile (the forward rule) and
+// v' its slope; R0 is the radar's fixed dechirp reference.
+//
+// The interpolated profile values are recomputed per pixel and pulse with
+// the same tile engine as the forward kernel (nothing per pixel-pulse is
+// stored).  Per (tile, pulse) the kernel reduces
+//   A = sum c,  B = sum c (dR - dR_c),  Cx = sum c q_x,  Cy = sum c q_y
+// and uses (a - p)/|a - p| = uh_c (1 - d/R_c) - q/R_c + O(|q|^2/R_c^2)
+// (d = dR - dR_c, q = p - p_c) to get the tile's share
+//   g_n = uh_c (A - B/R_c) - (Cx, Cy, 0)/R_c.
+// Lanes accumulate 8 pulses x 4 sums in registers; a reduce-scatter
+// butterfly over the wave leaves one sum per lane, then 4 waves add in LDS.
+// Tile shares land in partial[n][tile] (FP64) and are summed in tile order
+// by a second kernel (deterministic).
+#include <algorithm>
+#include <cmath>
+
+#include "sarf_bp_core.hpp"
+
+namespace sarf {
+
+int bp_pick_tile(const sarf_bp_desc *d, double dx, double dy);
+int bp_tile_shape(int id, int *tx, int *ty);
+int bp_fill_args(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, BpArgs *a);
+
+namespace {
+
+constexpr int kSub = 8;  // pulses reduced together
+
+// Reduce-scatter of 32 values over a wave64: afterwards lane l holds the
+// wave sum of value index idx(l) (lanes l and l ^ 32 hold the same one).
+__device__ __forceinline__ float wave_reduce_scatter32(float (&v)[32], int lane) {
+#pragma unroll
+    for (int step = 0; step < 5; step++) {
+        const int o = 1 << step;
+        const int half = 16 >> step;
+        const bool up = (lane & o) != 0;
+#pragma unroll
+        for (int i = 0; i < half; i++) {
+            const float keep = up ? v[i + half] : v[i];
+            const float send = up ? v[i] : v[i + half];
+            v[i] = keep + __shfl_xor(send, o, 64);
+        }
+    }
+    return v[0] + __shfl_xor(v[0], 32, 64);
+}
+
+__device__ __forceinline__ int scatter_index(int lane) {
+    return ((lane & 1) << 4) | ((lane & 2) << 2) | ((lane & 4)) | ((lane & 8) >> 2) |
+           ((lane & 16) >> 4);
+}
+
+template <int TY, int PX>
+__global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
+    BpArgs a, const float2 *__restrict__ prof, const double *__restrict__ ant,
+    const double *__restrict__ r0, const double *__restrict__ f0, const float2 *__restrict__ img,
+    const double *__restrict__ ent, double *__restrict__ partial) {
+    __shared__ BpLds L;
+    __shared__ float uz[kBpChunk];
+    __shared__ float red[kBpThreads / 64][32];
+    const int ntiles = a.tiles_x * a.tiles_y;
+    const int t = swizzle_tile(blockIdx.x, ntiles);
+    BpTile<TY, PX> T;
+    T.init(a, t);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+
+    // per-pixel forward image and entropy weight 2 w_p
+    const double E = ent[0], S = ent[1];
+    const float invS = (float)(1.0 / S), lnS = (float)log(S), Ef = (float)E;
+    float Ir[PX], Ii[PX], wg[PX];
+#pragma unroll
+    for (int j = 0; j < PX; j++) {
+        int x, y;
+        Ir[j] = Ii[j] = wg[j] = 0.f;
+        if (T.pixel(j, x, y, a)) {
+            const float2 v = img[(size_t)x * a.Ny + y];
+            const float m = v.x * v.x + v.y * v.y;
+            Ir[j] = v.x;
+            Ii[j] = v.y;
+            wg[j] = m > 0.f ? -2.f * (__logf(m) - lnS + Ef) * invS : 0.f;
+        }
+    }
+    const float invbin = a.invbin;
+    const float two_pi = 6.283185307179586f;
+    const PlainPos pos{ant};
+
+    for (int n0 = 0; n0 < a.Np;) {
+        T.stage(a, L, prof, pos, r0, f0, n0, min(kBpChunk, a.Np - n0));
+        // uh_z for the chunk (stage() keeps uh_x, uh_y, 1/(2R_c) in pA)
+        if (threadIdx.x < L.nc) {
+            const int n = n0 + threadIdx.x;
+            const double ux = ant[3 * n] - T.pcx, uy = ant[3 * n + 1] - T.pcy, zz = ant[3 * n + 2];
+            uz[threadIdx.x] = (float)(zz / sqrt(ux * ux + uy * uy + zz * zz));
+        }
+        __syncthreads();
+        const int nc = L.nc;
+        for (int c0 = 0; c0 < nc; c0 += kSub) {
+            float acc[32];
+#pragma unroll
+            for (int i = 0; i < 32; i++) acc[i] = 0.f;
+#pragma unroll
+            for (int k = 0; k < kSub; k++) {
+                const int c = c0 + k;
+                if (c < nc) {
+                    const float4 A = L.pA[c];
+                    const float4 B = L.pB[c];
+                    const int4 I = L.pI[c];
+                    const float sy = A.y * T.qy;
+                    const float wmax = B.w;
+                    const float kk = two_pi * B.z;  // rad per metre
+                    const float glo = (float)(-I.x), ghi = (float)(a.Nfft - 1 - I.x);
+                    float sA = 0.f, sB = 0.f, sC = 0.f;
+#pragma unroll
+                    for (int j = 0; j < PX; j++) {
+                        const float s = fmaf(A.x, T.qx[j], sy);
+                        const float tq = fmaf(-s, s, T.q2[j]);
+                        const float g = fmaf(s, A.w, A.z);
+                        const float dl = fmaf(tq, g, -s);
+                        const float p = fmaf(dl, invbin, B.x);
+                        const float fr = __builtin_amdgcn_fractf(p);
+                        const int i = (int)__builtin_amdgcn_fmed3f(p, 0.f, wmax);
+                        const float4 w = L.win[I.y + i];
+                        const float vr = fmaf(fr, w.z, w.x);
+                        const float vi = fmaf(fr, w.w, w.y);
+                        const float ph = fmaf(dl, B.z, B.y);
+                        const float sn = __builtin_amdgcn_sinf(ph);
+                        const float cs = __builtin_amdgcn_cosf(ph);
+                        // z = v' + j k v
+                        const float zr = fmaf(-kk, vi, w.z * invbin);
+                        const float zi = fmaf(kk, vr, w.w * invbin);
+                        const float er = fmaf(zr, cs, -zi * sn);
+                        const float ei = fmaf(zr, sn, zi * cs);
+                        float cpn = wg[j] * fmaf(Ir[j], er, Ii[j] * ei);
+                        if (!I.w) cpn = (p > glo && p < ghi) ? cpn : 0.f;
+                        sA += cpn;
+                        sB = fmaf(cpn, dl, sB);
+                        sC = fmaf(cpn, T.qx[j], sC);
+                    }
+                    acc[4 * k + 0] = sA;
+                    acc[4 * k + 1] = sB;
+                    acc[4 * k + 2] = sC;
+                    acc[4 * k + 3] = sA * T.qy;
+                }
+            }
+            const float r = wave_reduce_scatter32(acc, lane);
+            __syncthreads();  // previous users of red are done
+            if (lane < 32) red[wave][scatter_index(lane)] = r;
+            __syncthreads();
+            if (threadIdx.x < kSub && c0 + (int)threadIdx.x < nc) {
+                const int k = threadIdx.x, c = c0 + k;
+                float s4[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    float v = 0.f;
+                    for (int w = 0; w < kBpThreads / 64; w++) v += red[w][4 * k + q];
+                    s4[q] = v;
+                }
+                const float4 A = L.pA[c];
+                const double invR = 2.0 * (double)A.z;
+                const double base = (double)s4[0] - (double)s4[1] * invR;
+                double *o = partial + ((size_t)(n0 + c) * ntiles + t) * 3;
+                o[0] = (double)A.x * base - (double)s4[2] * invR;
+                o[1] = (double)A.y * base - (double)s4[3] * invR;
+                o[2] = (double)uz[c] * base;
+            }
+        }
+        n0 += nc;
+    }
+}
+
+// grad[n] = sum over tiles (in tile order) of partial[n][tile].
+__global__ __launch_bounds__(64) void adjoint_reduce(const double *__restrict__ partial,
+                                                     int ntiles, int Np,
+                                                     double *__restrict__ grad) {
+    const int n = blockIdx.x;
+    const int lane = threadIdx.x;
+    double g[3] = {0.0, 0.0, 0.0};
+    for (int t = lane; t < ntiles; t += 64) {
+        const double *p = partial + ((size_t)n * ntiles + t) * 3;
+        g[0] += p[0];
+        g[1] += p[1];
+        g[2] += p[2];
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) g[q] += __shfl_xor(g[q], o, 64);
+    }
+    if (lane == 0) {
+        grad[3 * n] = g[0];
+        grad[3 * n + 1] = g[1];
+        grad[3 * n + 2] = g[2];
+    }
+}
+
+template <int TY, int PX>
+void launch_adj_t(const BpArgs &a, int grid, const float2 *prof, const double *ant,
+                  const double *r0, const double *f0, const float2 *img, const double *ent,
+                  double *partial, hipStream_t s) {
+    hipLaunchKernelGGL((bp_entropy_adjoint<TY, PX>), dim3(grid), dim3(kBpThreads), 0, s, a, prof,
+                       ant, r0, f0, img, ent, partial);
+}
+
+}  // namespace
+}  // namespace sarf
+
+using namespace sarf;
+
+extern "C" int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof,
+                                    const double *ant, const double *r0, const double *f0,
+                                    const void *img, const double *ent, double *grad,
+                                    void *stream) {
+    clear_error();
+    SARF_TRY(use_device(ctx));
+    if (!d || !prof || !ant || !r0 || !f0 || !img || !ent || !grad)
+        return fail(SARF_ERR_INVALID, "sarf_bp_entropy_grad: null argument");
+    if (d->precision != SARF_BP_MIXED)
+        return fail(SARF_ERR_UNSUPPORTED, "sarf_bp_entropy_grad: mixed precision only");
+    BpArgs a{};
+    const int tile = d->tile > 0 ? d->tile : 0;
+    SARF_TRY(bp_fill_args(ctx, d, tile, &a));
+    int tx = 0, ty = 0;
+    const int tid = bp_tile_shape(tile > 0 ? tile : bp_pick_tile(d, a.dx, a.dy), &tx, &ty);
+    const int ntiles = a.tiles_x * a.tiles_y;
+    void *buf = nullptr;
+    SARF_TRY(scratch_get(ctx, "adjoint.partials", sizeof(double) * 3 * (size_t)ntiles * d->Np, &buf));
+    double *partial = (double *)buf;
+    hipStream_t s = (hipStream_t)stream;
+    const float2 *p = (const float2 *)prof;
+    const float2 *im = (const float2 *)img;
+    switch (tid) {
+        case 1: launch_adj_t<32, 4>(a, ntiles, p, ant, r0, f0, im, ent, partial, s); break;
+        case 2: launch_adj_t<64, 4>(a, ntiles, p, ant, r0, f0, im, ent, partial, s); break;
+        case 3: launch_adj_t<64, 8>(a, ntiles, p, ant, r0, f0, im, ent, partial, s); break;
+        case 4: launch_adj_t<32, 8>(a, ntiles, p, ant, r0, f0, im, ent, partial, s); break;
+        case 5: launch_adj_t<16, 4>(a, ntiles, p, ant, r0, f0, im, ent, partial, s); break;
+        default:
+            return fail(SARF_ERR_UNSUPPORTED, "sarf_bp_entropy_grad: tile %d not compiled", tid);
+    }
+    SARF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(adjoint_reduce, dim3(d->Np), dim3(64), 0, s, partial, ntiles, d->Np, grad);
+    SARF_HIP(hipGetLastError());
+    return SARF_OK;
+}
