@@ -159,12 +159,30 @@ void orc_range_bins_f32(int nfft, float maxWr, float *r) {
 /* Optional taper (per-sample real weight, matlab/rangeCompress.m:6-12) and  */
 /* phase ramp exp(-j 4 pi f0 R0 / c) per pulse (rangeCompress.m:17).         */
 /* ------------------------------------------------------------------------ */
+/* Unscaled inverse DFT of length n by definition (any n), O(n^2). */
+static void dft_inverse_direct(double *x, int n) {
+    double *y = (double *)calloc(2 * (size_t)n, sizeof(double));
+    for (int m = 0; m < n; m++) {
+        double ar = 0, ai = 0;
+        for (int k = 0; k < n; k++) {
+            double a = 2.0 * ORC_PI * (double)(((long)k * m) % n) / n;
+            double c = cos(a), s = sin(a);
+            ar += x[2 * k] * c - x[2 * k + 1] * s;
+            ai += x[2 * k] * s + x[2 * k + 1] * c;
+        }
+        y[2 * m] = ar;
+        y[2 * m + 1] = ai;
+    }
+    memcpy(x, y, sizeof(double) * 2 * n);
+    free(y);
+}
+
 int orc_range_compress_f64(int K, int Np, int nfft, const double *ph,
                            const double *taper /* K or NULL */,
                            const double *ramp_f0 /* Np or NULL */,
                            const double *ramp_r0 /* Np or NULL */,
                            double *prof) {
-    if (!is_pow2(nfft) || K > nfft || K <= 0 || Np <= 0) return -1;
+    if (K > nfft || K <= 0 || Np <= 0) return -1;
     int shift = (nfft + 1) / 2;
     int err = 0;
 #pragma omp parallel
@@ -178,7 +196,10 @@ int orc_range_compress_f64(int K, int Np, int nfft, const double *ph,
                 buf[2 * k] = ph[2 * ((size_t)n * K + k)] * w;
                 buf[2 * k + 1] = ph[2 * ((size_t)n * K + k) + 1] * w;
             }
-            orc_fft_f64(buf, nfft, +1);
+            if (is_pow2(nfft))
+                orc_fft_f64(buf, nfft, +1);
+            else
+                dft_inverse_direct(buf, nfft);
             double rr = 1.0 / nfft, ri = 0.0;
             if (ramp_f0 && ramp_r0) {
                 double a = -4.0 * ORC_PI * ramp_f0[n] * ramp_r0[n] / ORC_CLIGHT;

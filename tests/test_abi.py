@@ -64,3 +64,18 @@ def test_invalid_context_argument_is_an_error_code():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(ImportError):
         _lib.load(str(tmp_path / "nope.so"))
+
+
+def test_cpp_interface_headers_compile_with_plain_gcc(tmp_path):
+    """include/*.hpp (the reference's C++ surface over the C ABI) needs only a
+    C++17 compiler and sarf.h -- no HIP headers -- to compile caller code."""
+    import shutil
+    import subprocess
+
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    src = os.path.join(REPO, "uncc_sar_focusing_amd", "examples", "example_gpu.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
+                        "-I" + os.path.join(REPO, "include"), src],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout

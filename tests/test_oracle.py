@@ -51,6 +51,19 @@ def test_range_compress_matches_numpy_ifft_fftshift():
     np.testing.assert_allclose(prof, ref, atol=1e-14)
 
 
+@pytest.mark.parametrize("nfft", [37, 424, 500])
+def test_range_compress_any_length_matches_numpy(nfft):
+    """Non-power-of-two Nfft (the reference default N_fft = K) by direct DFT;
+    fftshift = cshift((N+1)/2) also for odd N (cpuBackProjection_fft.hpp:81-84)."""
+    rng = np.random.default_rng(nfft)
+    K = min(nfft, 300)
+    ph = rng.standard_normal((3, K)) + 1j * rng.standard_normal((3, K))
+    prof = oracle.range_compress_f64(ph, nfft)
+    x = np.fft.ifft(ph, nfft, axis=1)
+    ref = np.roll(x, -((nfft + 1) // 2), axis=1)  # valarray::cshift((N+1)/2)
+    np.testing.assert_allclose(prof, ref, atol=1e-12)
+
+
 def test_range_compress_point_target_closed_form():
     """A point at differential range d gives the Dirichlet kernel
     (1/N) sum_k e^{-j 4 pi f_k d/c} e^{+j 2 pi k (m - N/2)/N}, peaked at

@@ -65,8 +65,7 @@ int scratch_get(sarf_ctx *ctx, const char *name, size_t bytes, void **out) {
 }
 
 template <typename T2>
-static int make_twiddles(int log2n, T2 **out) {
-    const long n = 1L << log2n;
+static int make_twiddles(long n, T2 **out) {
     std::vector<T2> h(n);
     for (long i = 0; i < n; i++) {
         // e^{+j 2 pi i / N}, evaluated in double and rounded once
@@ -82,28 +81,36 @@ static int make_twiddles(int log2n, T2 **out) {
     return SARF_OK;
 }
 
-int twiddles_f32(sarf_ctx *ctx, int log2n, const float2 **out) {
+int twiddles_any_f32(sarf_ctx *ctx, long n, const float2 **out) {
     std::lock_guard<std::mutex> lk(ctx->mu);
-    auto it = ctx->tw32.find(log2n);
+    auto it = ctx->tw32.find(n);
     if (it == ctx->tw32.end()) {
         float2 *d = nullptr;
-        SARF_TRY(make_twiddles(log2n, &d));
-        it = ctx->tw32.emplace(log2n, d).first;
+        SARF_TRY(make_twiddles(n, &d));
+        it = ctx->tw32.emplace(n, d).first;
     }
     *out = it->second;
     return SARF_OK;
 }
 
-int twiddles_f64(sarf_ctx *ctx, int log2n, const double2 **out) {
+int twiddles_any_f64(sarf_ctx *ctx, long n, const double2 **out) {
     std::lock_guard<std::mutex> lk(ctx->mu);
-    auto it = ctx->tw64.find(log2n);
+    auto it = ctx->tw64.find(n);
     if (it == ctx->tw64.end()) {
         double2 *d = nullptr;
-        SARF_TRY(make_twiddles(log2n, &d));
-        it = ctx->tw64.emplace(log2n, d).first;
+        SARF_TRY(make_twiddles(n, &d));
+        it = ctx->tw64.emplace(n, d).first;
     }
     *out = it->second;
     return SARF_OK;
+}
+
+int twiddles_f32(sarf_ctx *ctx, int log2n, const float2 **out) {
+    return twiddles_any_f32(ctx, 1L << log2n, out);
+}
+
+int twiddles_f64(sarf_ctx *ctx, int log2n, const double2 **out) {
+    return twiddles_any_f64(ctx, 1L << log2n, out);
 }
 
 }  // namespace sarf

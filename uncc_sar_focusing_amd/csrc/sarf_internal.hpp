@@ -48,9 +48,9 @@ struct sarf_ctx {
     int device = 0;
     int cu_count = 0;
     std::mutex mu;
-    // FFT twiddle tables e^{+j 2 pi i / N}, keyed by log2(N)
-    std::map<int, float2 *> tw32;
-    std::map<int, double2 *> tw64;
+    // DFT twiddle tables e^{+j 2 pi i / N}, keyed by N
+    std::map<long, float2 *> tw32;
+    std::map<long, double2 *> tw64;
     // named scratch buffers (partial images, reductions, staging)
     std::map<std::string, sarf::Scratch> scratch;
 };
@@ -64,6 +64,8 @@ int scratch_get(sarf_ctx *ctx, const char *name, size_t bytes, void **out);
 // Twiddle table for N = 2^log2n (created on first use, synchronously).
 int twiddles_f32(sarf_ctx *ctx, int log2n, const float2 **out);
 int twiddles_f64(sarf_ctx *ctx, int log2n, const double2 **out);
+int twiddles_any_f32(sarf_ctx *ctx, long n, const float2 **out);
+int twiddles_any_f64(sarf_ctx *ctx, long n, const double2 **out);
 
 int ilog2_exact(long n);  // log2 if n is a power of two, else -1
 

@@ -295,6 +295,65 @@ __global__ __launch_bounds__(256) void rc_pass_f64(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Any Nfft (not a power of two, e.g. the reference's default N_fft = K,
+// charlotte_sar_api.hpp:173): direct inverse DFT, O(K) per output bin.
+//   prof[m] = (1/N) sum_{k<K} x_k e^{+j 2 pi k m' / N},  m' = (m + (N+1)/2) mod N
+// The twiddle index k m' mod N is carried exactly in integers and looked up
+// in the e^{+j 2 pi i / N} table, so no phase error accumulates.  A workgroup
+// stages one pulse's K samples in LDS (chunks of 2048) and each lane forms
+// one output bin.
+// ---------------------------------------------------------------------------
+template <typename T2, typename R>
+__global__ __launch_bounds__(256) void rc_direct(const T2 *__restrict__ ph, int K, int N,
+                                                 int flags, const float *__restrict__ taper,
+                                                 const double *__restrict__ f0,
+                                                 const double *__restrict__ r0,
+                                                 const T2 *__restrict__ tw, T2 *__restrict__ prof) {
+    constexpr int kChunk = 2048;
+    __shared__ T2 xs[kChunk];
+    const int n = blockIdx.y;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int mp = (int)(((long)m + (N + 1) / 2) % N);
+    R ar = 0, ai = 0;
+    for (int k0 = 0; k0 < K; k0 += kChunk) {
+        const int kn = min(kChunk, K - k0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < kn; i += blockDim.x) {
+            T2 x = ph[(size_t)n * K + k0 + i];
+            if (flags & SARF_RC_TAPER) {
+                x.x *= taper[k0 + i];
+                x.y *= taper[k0 + i];
+            }
+            xs[i] = x;
+        }
+        __syncthreads();
+        if (m < N) {
+            long idx = ((long)k0 * mp) % N;  // (k m') mod N for k = k0
+            for (int i = 0; i < kn; i++) {
+                const T2 w = tw[idx];
+                const T2 x = xs[i];
+                ar += (R)x.x * w.x - (R)x.y * w.y;
+                ai += (R)x.x * w.y + (R)x.y * w.x;
+                idx += mp;
+                if (idx >= N) idx -= N;
+            }
+        }
+    }
+    if (m < N) {
+        double sr = 1.0 / N, si = 0.0;
+        if (flags & SARF_RC_RAMP) {
+            const double a = -4.0 * kPi * f0[n] * r0[n] / kClight;
+            sr = cos(a) / N;
+            si = sin(a) / N;
+        }
+        T2 o;
+        o.x = (decltype(o.x))(ar * (R)sr - ai * (R)si);
+        o.y = (decltype(o.y))(ar * (R)si + ai * (R)sr);
+        prof[(size_t)n * N + m] = o;
+    }
+}
+
 template <int LOG2N>
 int launch_lds(const sarf_rc_desc *d, const float2 *ph, const float *taper, const double *f0,
                const double *r0, const float2 *tw, float2 *prof, hipStream_t s) {
@@ -314,13 +373,35 @@ int launch_range_compress(sarf_ctx *ctx, const sarf_rc_desc *d, const void *ph,
                           const float *taper, const double *f0, const double *r0,
                           void *prof, hipStream_t s) {
     const int log2n = ilog2_exact(d->Nfft);
-    if (d->K < 1 || d->Np < 1 || d->K > d->Nfft || log2n < 0)
-        return fail(SARF_ERR_INVALID, "range_compress: need 1 <= K <= Nfft, Nfft a power of two "
+    if (d->K < 1 || d->Np < 1 || d->K > d->Nfft)
+        return fail(SARF_ERR_INVALID, "range_compress: need 1 <= K <= Nfft, Np >= 1 "
                     "(K=%d Np=%d Nfft=%d)", d->K, d->Np, d->Nfft);
     if ((d->flags & SARF_RC_TAPER) && !taper)
         return fail(SARF_ERR_INVALID, "range_compress: SARF_RC_TAPER without taper");
     if ((d->flags & SARF_RC_RAMP) && (!f0 || !r0))
         return fail(SARF_ERR_INVALID, "range_compress: SARF_RC_RAMP without f0/r0");
+    if (log2n < 0) {
+        // not a power of two: direct DFT with an exact twiddle index
+        if (d->Nfft > (1 << 20) || d->Np > 65535)
+            return fail(SARF_ERR_UNSUPPORTED, "range_compress: Nfft=%d / Np=%d too large for the "
+                        "direct transform", d->Nfft, d->Np);
+        dim3 grid((d->Nfft + 255) / 256, d->Np);
+        if (d->flags & SARF_RC_F64) {
+            const double2 *tw = nullptr;
+            SARF_TRY(twiddles_any_f64(ctx, d->Nfft, &tw));
+            hipLaunchKernelGGL((rc_direct<double2, double>), grid, dim3(256), 0, s,
+                               (const double2 *)ph, d->K, d->Nfft, d->flags, taper, f0, r0, tw,
+                               (double2 *)prof);
+        } else {
+            const float2 *tw = nullptr;
+            SARF_TRY(twiddles_any_f32(ctx, d->Nfft, &tw));
+            hipLaunchKernelGGL((rc_direct<float2, float>), grid, dim3(256), 0, s,
+                               (const float2 *)ph, d->K, d->Nfft, d->flags, taper, f0, r0, tw,
+                               (float2 *)prof);
+        }
+        SARF_HIP(hipGetLastError());
+        return SARF_OK;
+    }
     if (d->flags & SARF_RC_F64) {
         if (log2n < 1 || log2n > 24)
             return fail(SARF_ERR_UNSUPPORTED, "range_compress f64: Nfft=%d", d->Nfft);
