@@ -17,10 +17,11 @@
 // and uses (a - p)/|a - p| = uh_c (1 - d/R_c) - q/R_c + O(|q|^2/R_c^2)
 // (d = dR - dR_c, q = p - p_c) to get the tile's share
 //   g_n = uh_c (A - B/R_c) - (Cx, Cy, 0)/R_c.
-// Lanes accumulate 8 pulses x 4 sums in registers; a reduce-scatter
+// Lanes accumulate 4 pulses x 4 sums in registers; a reduce-scatter
 // butterfly over the wave leaves one sum per lane, then 4 waves add in LDS.
 // Tile shares land in partial[n][tile] (FP64) and are summed in tile order
-// by a second kernel (deterministic).
+// by a second kernel (deterministic).  Pulse ranges are split over
+// workgroups when the image has too few tiles to fill the GPU.
 #include <algorithm>
 #include <cmath>
 
@@ -34,15 +35,18 @@ int bp_fill_args(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, BpArgs *a);
 
 namespace {
 
-constexpr int kSub = 8;  // pulses reduced together
+constexpr int kSub = 4;          // pulses reduced together
+constexpr int kNv = 4 * kSub;    // sums per lane per group (A, B, Cx, Cy per pulse)
+constexpr int kLogNv = 4;
 
-// Reduce-scatter of 32 values over a wave64: afterwards lane l holds the
-// wave sum of value index idx(l) (lanes l and l ^ 32 hold the same one).
-__device__ __forceinline__ float wave_reduce_scatter32(float (&v)[32], int lane) {
+// Reduce-scatter of kNv values over a wave64: after log2(kNv) halving steps
+// lane l holds the sum over a group of kNv lanes of value index idx(l); two
+// more xor steps finish the wave sum (replicated in 64/kNv lanes).
+__device__ __forceinline__ float wave_reduce_scatter(float (&v)[kNv], int lane) {
 #pragma unroll
-    for (int step = 0; step < 5; step++) {
+    for (int step = 0; step < kLogNv; step++) {
         const int o = 1 << step;
-        const int half = 16 >> step;
+        const int half = (kNv / 2) >> step;
         const bool up = (lane & o) != 0;
 #pragma unroll
         for (int i = 0; i < half; i++) {
@@ -51,12 +55,20 @@ __device__ __forceinline__ float wave_reduce_scatter32(float (&v)[32], int lane)
             v[i] = keep + __shfl_xor(send, o, 64);
         }
     }
-    return v[0] + __shfl_xor(v[0], 32, 64);
+    float r = v[0];
+#pragma unroll
+    for (int o = kNv; o < 64; o <<= 1) r += __shfl_xor(r, o, 64);
+    return r;
 }
 
+// Value index held by lane l (< kNv) after wave_reduce_scatter: bit b of l
+// (step b kept the upper half) has weight kNv / 2^(b+1).
 __device__ __forceinline__ int scatter_index(int lane) {
-    return ((lane & 1) << 4) | ((lane & 2) << 2) | ((lane & 4)) | ((lane & 8) >> 2) |
-           ((lane & 16) >> 4);
+    int idx = 0;
+#pragma unroll
+    for (int b = 0; b < kLogNv; b++)
+        if (lane & (1 << b)) idx += (kNv >> 1) >> b;
+    return idx;
 }
 
 template <int TY, int PX>
@@ -66,9 +78,14 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
     const double *__restrict__ ent, double *__restrict__ partial) {
     __shared__ BpLds L;
     __shared__ float uz[kBpChunk];
-    __shared__ float red[kBpThreads / 64][32];
+    __shared__ float red[kBpThreads / 64][kNv];
+    // blockIdx.x = split * ntiles + tile; each split covers a pulse range
     const int ntiles = a.tiles_x * a.tiles_y;
-    const int t = swizzle_tile(blockIdx.x, ntiles);
+    const int split = blockIdx.x / ntiles;
+    const int t = swizzle_tile(blockIdx.x - split * ntiles, ntiles);
+    const int per_split = (a.Np + a.splits - 1) / a.splits;
+    const int nbeg = min(a.Np, split * per_split);
+    const int nend = min(a.Np, nbeg + per_split);
     BpTile<TY, PX> T;
     T.init(a, t);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -93,8 +110,8 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
     const float two_pi = 6.283185307179586f;
     const PlainPos pos{ant};
 
-    for (int n0 = 0; n0 < a.Np;) {
-        T.stage(a, L, prof, pos, r0, f0, n0, min(kBpChunk, a.Np - n0));
+    for (int n0 = nbeg; n0 < nend;) {
+        T.stage(a, L, prof, pos, r0, f0, n0, min(kBpChunk, nend - n0));
         // uh_z for the chunk (stage() keeps uh_x, uh_y, 1/(2R_c) in pA)
         if (threadIdx.x < L.nc) {
             const int n = n0 + threadIdx.x;
@@ -104,9 +121,9 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
         __syncthreads();
         const int nc = L.nc;
         for (int c0 = 0; c0 < nc; c0 += kSub) {
-            float acc[32];
+            float acc[kNv];
 #pragma unroll
-            for (int i = 0; i < 32; i++) acc[i] = 0.f;
+            for (int i = 0; i < kNv; i++) acc[i] = 0.f;
 #pragma unroll
             for (int k = 0; k < kSub; k++) {
                 const int c = c0 + k;
@@ -151,9 +168,9 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
                     acc[4 * k + 3] = sA * T.qy;
                 }
             }
-            const float r = wave_reduce_scatter32(acc, lane);
+            const float r = wave_reduce_scatter(acc, lane);
             __syncthreads();  // previous users of red are done
-            if (lane < 32) red[wave][scatter_index(lane)] = r;
+            if (lane < kNv) red[wave][scatter_index(lane)] = r;
             __syncthreads();
             if (threadIdx.x < kSub && c0 + (int)threadIdx.x < nc) {
                 const int k = threadIdx.x, c = c0 + k;
@@ -225,24 +242,43 @@ extern "C" int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *d, const 
         return fail(SARF_ERR_INVALID, "sarf_bp_entropy_grad: null argument");
     if (d->precision != SARF_BP_MIXED)
         return fail(SARF_ERR_UNSUPPORTED, "sarf_bp_entropy_grad: mixed precision only");
+    // default: the forward kernel's choice, narrowed to 4 pixels per lane
+    // (tile 4 -> 5, 3 -> 2) to keep the adjoint's registers in check
+    int tile = d->tile;
+    if (tile <= 0) {
+        BpArgs tmp{};
+        SARF_TRY(bp_fill_args(ctx, d, 0, &tmp));
+        tile = bp_pick_tile(d, tmp.dx, tmp.dy);
+        if (tile == 4) tile = 5;
+        if (tile == 3) tile = 2;
+    }
     BpArgs a{};
-    const int tile = d->tile > 0 ? d->tile : 0;
     SARF_TRY(bp_fill_args(ctx, d, tile, &a));
     int tx = 0, ty = 0;
-    const int tid = bp_tile_shape(tile > 0 ? tile : bp_pick_tile(d, a.dx, a.dy), &tx, &ty);
+    const int tid = bp_tile_shape(tile, &tx, &ty);
     const int ntiles = a.tiles_x * a.tiles_y;
+    // Fill the GPU: pulse groups share nothing (partials are per tile and
+    // pulse), so split until there are ~16 workgroups per CU (measured best
+    // at 512^2 x 1024), >= 32 pulses each.
+    int splits = d->pulse_split > 0 ? d->pulse_split : 1;
+    if (d->pulse_split <= 0) {
+        const int want = 16 * (ctx->cu_count > 0 ? ctx->cu_count : 256);
+        while (splits < 64 && ntiles * splits < want && d->Np / (2 * splits) >= 32) splits *= 2;
+    }
+    a.splits = std::max(1, std::min(splits, d->Np));
     void *buf = nullptr;
     SARF_TRY(scratch_get(ctx, "adjoint.partials", sizeof(double) * 3 * (size_t)ntiles * d->Np, &buf));
     double *partial = (double *)buf;
     hipStream_t s = (hipStream_t)stream;
     const float2 *p = (const float2 *)prof;
     const float2 *im = (const float2 *)img;
+    const int grid = ntiles * a.splits;
     switch (tid) {
-        case 1: launch_adj_t<32, 4>(a, ntiles, p, ant, r0, f0, im, ent, partial, s); break;
-        case 2: launch_adj_t<64, 4>(a, ntiles, p, ant, r0, f0, im, ent, partial, s); break;
-        case 3: launch_adj_t<64, 8>(a, ntiles, p, ant, r0, f0, im, ent, partial, s); break;
-        case 4: launch_adj_t<32, 8>(a, ntiles, p, ant, r0, f0, im, ent, partial, s); break;
-        case 5: launch_adj_t<16, 4>(a, ntiles, p, ant, r0, f0, im, ent, partial, s); break;
+        case 1: launch_adj_t<32, 4>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
+        case 2: launch_adj_t<64, 4>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
+        case 3: launch_adj_t<64, 8>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
+        case 4: launch_adj_t<32, 8>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
+        case 5: launch_adj_t<16, 4>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
         default:
             return fail(SARF_ERR_UNSUPPORTED, "sarf_bp_entropy_grad: tile %d not compiled", tid);
     }

@@ -43,6 +43,48 @@ def synth(ctx, geom: Geometry, tg: Targets, dev="cuda:0", out=None) -> torch.Ten
                                    _dev(tg.amp.astype(np.complex64), dev), out=out)
 
 
+def smoothed_random_walk(Np: int, sigma: float, seed: int, stream: int = 21,
+                         smooth: float = 16.0) -> np.ndarray:
+    """(Np, 3) per-pulse position errors: a Gaussian random walk per axis,
+    smoothed by a Gaussian of `smooth` pulses, mean removed, scaled to
+    standard deviation `sigma` (m) per axis."""
+    from .scenes import normal
+
+    k = np.arange(-int(4 * smooth), int(4 * smooth) + 1)
+    ker = np.exp(-0.5 * (k / smooth) ** 2)
+    ker /= ker.sum()
+    out = np.empty((Np, 3))
+    for ax in range(3):
+        w = np.cumsum(normal(seed, stream + ax, Np))
+        w = np.convolve(np.pad(w, len(k) // 2, mode="edge"), ker, mode="valid")
+        w -= w.mean()
+        out[:, ax] = w * (sigma / w.std())
+    return out
+
+
+def random_walk_scene(ctx, name: str = "C2", sigma: float = 0.05, seed: int | None = None,
+                      clutter: bool = True, dev: str = "cuda:0") -> DeviceScene:
+    """The grid-search scene with the error replaced by independent per-pulse
+    3-D position errors (smoothed random walk, sigma per axis)."""
+    cfg = CONFIGS[name]
+    seed = cfg.seed if seed is None else seed
+    geom = cfg.geometry()
+    grid = cfg.grid()
+    tg = cfg.targets()
+    err = smoothed_random_walk(geom.Np, sigma, seed)
+    true_geom = Geometry(ant=geom.ant + err, r0=geom.r0.copy(), f0=geom.f0.copy(), df=geom.df,
+                         K=geom.K)
+    ph = synth(ctx, true_geom, tg, dev)
+    n_cl = 0
+    if clutter and cfg.clutter_stride:
+        cl = grid_clutter(grid, cfg.clutter_stride, cfg.clutter_db, seed)
+        synth(ctx, true_geom, cl, dev, out=ph)
+        n_cl = cl.pos.shape[0]
+    torch.cuda.synchronize()
+    return DeviceScene(cfg, geom, true_geom, grid, tg, ph, _dev(geom.ant, dev), _dev(geom.r0, dev),
+                       _dev(geom.f0, dev), true_coef=None, n_clutter=n_cl)
+
+
 def los_error_scene(ctx, name: str = "C2", seed: int | None = None, clutter: bool = True,
                     dev: str = "cuda:0", coef_range: float = 0.3) -> DeviceScene:
     cfg = CONFIGS[name]
