@@ -179,6 +179,50 @@ int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *desc,
                          void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Backprojection of many independent scenes of uneven size in one          */
+/* persistent launch (work items = scene tiles, largest first, drained      */
+/* through an atomic counter).  Each scene is described as for              */
+/* sarf_backproject, with its own device buffers; images are overwritten.   */
+/* tile: 0 = chosen for the finest-spaced scene, else a tile id 1..5.       */
+/* The reference focuses one aperture per call (cuda_focus_SAR_image,       */
+/* cuda_sar_focusing.hpp:526-723).                                          */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    const void *prof;   /* device float2 Nfft x Np */
+    const double *ant;  /* device Np x 3 */
+    const double *r0;   /* device Np */
+    const double *f0;   /* device Np */
+    void *img;          /* device float2 Ny x Nx */
+    int Nx, Ny, Nfft, Np;
+    double x0, y0, Wx, Wy, maxWr;
+    double los_x, los_y; /* optional tile hint, as in sarf_bp_desc */
+} sarf_scene;
+
+int sarf_bp_scenes(sarf_ctx *ctx, int nscenes,
+                   const sarf_scene *scenes, /* host array of nscenes descriptors */
+                   int tile, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Phase gradient autofocus of a focused image, in place (matlab/PGA.m:     */
+/* 151-232; C++ restatement src/autofocus/cpu/autofocus_cpu.cpp:249-474).   */
+/* Rows are range (y), columns azimuth (x); img is img[x*Ny + y].  Runs a    */
+/* fixed number of iterations (no early stop).  af_ph (device, Nx doubles)  */
+/* receives the accumulated detrended phase estimate, rms (device, iters)   */
+/* the rms of each iteration's estimate.                                    */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int Nx, Ny;      /* image size; 2 <= Nx <= 4096 */
+    int iters;       /* iterations */
+    double db_down;  /* window cut-off below the peak column energy, dB (30) */
+} sarf_pga_desc;
+
+int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *desc,
+             void *img,       /* device float2 Ny x Nx, updated in place */
+             double *af_ph,   /* device, Nx */
+             double *rms,     /* device, iters */
+             void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Whole focusing call on host buffers: H2D, range compression,             */
 /* backprojection, D2H.  This is what cuda_focus_SAR_image<float,float>     */
 /* (cuda_sar_focusing.hpp:526-723) does; it is the entry the C++ host       */

@@ -233,3 +233,28 @@ def test_bp_pulse_split_partials_match_oracle(ctx, split):
     ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr, pulse_split=split, out=base,
                     accumulate=True)
     assert oracle.rel_l2(base.cpu().numpy() - (1 + 2j), ref) <= 1e-4
+
+
+def test_bp_scenes_uneven_batch_matches_single_calls(ctx):
+    """One persistent launch over scenes of different pulses, profile
+    lengths and image sizes equals backprojecting each scene alone, and the
+    FP64 oracle within 1e-4."""
+    specs = [(40, 64, 256, ImageGrid(37, 52, 0.1, -0.2, 4.0, 5.0)),
+             (97, 96, 1024, ImageGrid.square(64, 0.1)),
+             (16, 48, 512, ImageGrid(20, 90, 0.0, 0.3, 2.0, 9.0))]
+    scenes, refs = [], []
+    for i, (Np, K, nfft, grid) in enumerate(specs):
+        geom, tg, ph = small_inputs(Np=Np, K=K, seed=10 + i)
+        ant, r0, f0 = geom_dev(geom)
+        prof = ctx.range_compress(to_dev(ph), nfft)
+        scenes.append((prof, ant, r0, f0, grid, geom.maxWr))
+        refs.append((ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr).cpu().numpy(),
+                     oracle.bp_f64(oracle.range_compress_f64(ph, nfft), geom.ant, geom.r0,
+                                   geom.f0, grid, geom.maxWr)))
+    for tile in (0, 1, 5):
+        imgs = ctx.backproject_scenes(scenes, tile=tile)
+        for img, (single, ref) in zip(imgs, refs):
+            img = img.cpu().numpy()
+            assert oracle.rel_l2(img, ref) <= 1e-4
+            if tile == 0:
+                assert oracle.rel_l2(img, single) < 1e-5

@@ -74,7 +74,22 @@ SIGNATURES = {
     "sarf_bp_batch_entropy": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _i, _i, _vp, _vp,
                                    _vp, _vp, _vp]),
     "sarf_bp_entropy_grad": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sarf_pga": (_i, [_vp, C.c_void_p, _vp, _vp, _vp, _vp]),
+    "sarf_bp_scenes": (_i, [_vp, _i, C.c_void_p, _i, _vp]),
 }
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("prof", _vp), ("ant", _vp), ("r0", _vp), ("f0", _vp), ("img", _vp),
+        ("Nx", C.c_int), ("Ny", C.c_int), ("Nfft", C.c_int), ("Np", C.c_int),
+        ("x0", C.c_double), ("y0", C.c_double), ("Wx", C.c_double), ("Wy", C.c_double),
+        ("maxWr", C.c_double), ("los_x", C.c_double), ("los_y", C.c_double),
+    ]
+
+
+class PgaDesc(C.Structure):
+    _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("iters", C.c_int), ("db_down", C.c_double)]
 
 
 def load(path: str = LIB_PATH) -> C.CDLL:

@@ -182,6 +182,43 @@ class Context:
                                             self._stream()))
         return out
 
+    def backproject_scenes(self, scenes, *, tile: int = 0) -> list:
+        """Backproject many independent scenes in one persistent launch.
+        `scenes` is a sequence of (prof, ant, r0, f0, grid, maxWr) or
+        (prof, ant, r0, f0, grid, maxWr, img); returns the images."""
+        descs = (_lib.SceneDesc * len(scenes))()
+        keep = []
+        for i, sc in enumerate(scenes):
+            prof, ant, r0, f0, grid, maxWr = sc[:6]
+            img = sc[6] if len(sc) > 6 else torch.empty((grid.Nx, grid.Ny), dtype=torch.complex64,
+                                                        device=prof.device)
+            Np, nfft = prof.shape
+            _need(prof, torch.complex64, (Np, nfft), f"scene {i} prof")
+            _need(ant, torch.float64, (Np, 3), f"scene {i} ant")
+            _need(r0, torch.float64, (Np,), f"scene {i} r0")
+            _need(f0, torch.float64, (Np,), f"scene {i} f0")
+            _need(img, torch.complex64, (grid.Nx, grid.Ny), f"scene {i} img")
+            descs[i] = _lib.SceneDesc(prof.data_ptr(), ant.data_ptr(), r0.data_ptr(),
+                                      f0.data_ptr(), img.data_ptr(), grid.Nx, grid.Ny, nfft, Np,
+                                      grid.x0, grid.y0, grid.Wx, grid.Wy, maxWr, 0.0, 0.0)
+            keep.append(img)
+        check(self.lib.sarf_bp_scenes(self._h, len(scenes), descs, tile, self._stream()))
+        return keep
+
+    # ----------------------------------------------------------------- PGA
+    def pga(self, img: torch.Tensor, iters: int, db_down: float = 30.0):
+        """Phase gradient autofocus of img (Nx, Ny) complex64 in place
+        (PGA.m:151-232).  Returns (af_ph (Nx,) float64, rms (iters,) float64)
+        device tensors."""
+        Nx, Ny = img.shape
+        _need(img, torch.complex64, (Nx, Ny), "img")
+        af_ph = torch.empty(Nx, dtype=torch.float64, device=img.device)
+        rms = torch.empty(max(iters, 1), dtype=torch.float64, device=img.device)
+        d = _lib.PgaDesc(Nx, Ny, iters, db_down)
+        check(self.lib.sarf_pga(self._h, C.byref(d), _ptr(img), _ptr(af_ph), _ptr(rms),
+                                self._stream()))
+        return af_ph, rms[:iters]
+
     # --------------------------------------------------------------- focus
     def focus_host(self, ph, ant, r0, f0, grid: ImageGrid, nfft: int, maxWr: float,
                    precision: str = "mixed"):

@@ -1,0 +1,157 @@
+// sarf_scenes.hip -- backprojection of many independent scenes of uneven
+// size (pulses, profile length, image size, geometry) in one persistent
+// launch.
+//
+// The reference focuses one aperture per call (cuda_focus_SAR_image,
+// cuda_sar_focusing.hpp:526-723), allocating and resetting the device each
+// time.  Here a host-built list of (scene, tile) work items, sorted by cost
+// (pulses x tile pixels, largest first), is drained by a persistent grid
+// through an atomic work counter, so short and long scenes share the GPU
+// without a launch per scene; every item runs the same tile engine as
+// sarf_backproject and writes its tile of that scene's image.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "sarf_bp_core.hpp"
+
+namespace sarf {
+
+int bp_pick_tile(const sarf_bp_desc *d, double dx, double dy);
+int bp_tile_shape(int id, int *tx, int *ty);
+int bp_fill_args(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, BpArgs *a);
+
+namespace {
+
+struct SceneDev {
+    BpArgs a;
+    const float2 *prof;
+    const double *ant, *r0, *f0;
+    float2 *img;
+};
+
+__global__ void reset_counter(unsigned int *c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
+}
+
+template <int TY, int PX>
+__global__ __launch_bounds__(kBpThreads) void bp_scenes(const SceneDev *__restrict__ scenes,
+                                                        const int2 *__restrict__ items,
+                                                        int nitems,
+                                                        unsigned int *__restrict__ counter) {
+    __shared__ BpLds L;
+    __shared__ int sItem;
+    for (;;) {
+        __syncthreads();  // previous item done with sItem and L
+        if (threadIdx.x == 0) sItem = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int it = sItem;
+        if (it >= nitems) break;  // every workgroup reaches this once the list is drained
+        const int2 w = items[it];
+        const SceneDev &S = scenes[w.x];
+        const BpArgs a = S.a;
+        BpTile<TY, PX> T;
+        T.init(a, w.y);
+        T.run(a, L, S.prof, PlainPos{S.ant}, S.r0, S.f0, 0, a.Np);
+#pragma unroll
+        for (int j = 0; j < PX; j++) {
+            int x, y;
+            if (T.pixel(j, x, y, a)) S.img[(size_t)x * a.Ny + y] = make_float2(T.accr[j], T.acci[j]);
+        }
+    }
+}
+
+template <int TY, int PX>
+void launch_scenes_t(int grid, const SceneDev *sc, const int2 *items, int nitems,
+                     unsigned int *counter, hipStream_t s) {
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, s, counter);
+    hipLaunchKernelGGL((bp_scenes<TY, PX>), dim3(grid), dim3(kBpThreads), 0, s, sc, items, nitems,
+                       counter);
+}
+
+}  // namespace
+}  // namespace sarf
+
+using namespace sarf;
+
+extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scenes, int tile,
+                              void *stream) {
+    clear_error();
+    SARF_TRY(use_device(ctx));
+    if (nscenes < 1 || !scenes) return fail(SARF_ERR_INVALID, "sarf_bp_scenes: no scenes");
+    std::vector<SceneDev> host(nscenes);
+    struct Item {
+        int scene, tile;
+        double cost;
+    };
+    std::vector<Item> items;
+    // one tile shape for the whole batch: the given one, else the choice for
+    // the finest-spaced scene (it satisfies the precision bound for all)
+    int tid = tile;
+    if (tid <= 0) {
+        double best_sp = 1e300;
+        for (int i = 0; i < nscenes; i++) {
+            const sarf_scene &c = scenes[i];
+            if (c.Nx < 2 || c.Ny < 2) continue;
+            const double sp = std::max(c.Wx / (c.Nx - 1), c.Wy / (c.Ny - 1));
+            if (sp < best_sp) {
+                best_sp = sp;
+                sarf_bp_desc d{};
+                d.Nx = c.Nx; d.Ny = c.Ny; d.Nfft = c.Nfft; d.Np = c.Np;
+                d.x0 = c.x0; d.y0 = c.y0; d.Wx = c.Wx; d.Wy = c.Wy; d.maxWr = c.maxWr;
+                d.los_x = c.los_x; d.los_y = c.los_y;
+                tid = bp_pick_tile(&d, c.Wx / (c.Nx - 1), c.Wy / (c.Ny - 1));
+            }
+        }
+        // the batch kernel keeps 8 pixels per lane at most
+        if (tid == 6) tid = 3;
+        if (tid == 7) tid = 4;
+    }
+    int tx = 0, ty = 0;
+    if (!bp_tile_shape(tid, &tx, &ty) || tid > 5)
+        return fail(SARF_ERR_INVALID, "sarf_bp_scenes: tile %d not available for batches", tid);
+    for (int i = 0; i < nscenes; i++) {
+        const sarf_scene &c = scenes[i];
+        if (!c.prof || !c.ant || !c.r0 || !c.f0 || !c.img)
+            return fail(SARF_ERR_INVALID, "sarf_bp_scenes: scene %d has a null pointer", i);
+        sarf_bp_desc d{};
+        d.Nx = c.Nx; d.Ny = c.Ny; d.Nfft = c.Nfft; d.Np = c.Np;
+        d.x0 = c.x0; d.y0 = c.y0; d.Wx = c.Wx; d.Wy = c.Wy; d.maxWr = c.maxWr;
+        d.precision = SARF_BP_MIXED;
+        BpArgs a{};
+        const int rc = bp_fill_args(ctx, &d, tid, &a);
+        if (rc != SARF_OK) return fail(rc, "sarf_bp_scenes: scene %d: %s", i, sarf_last_error());
+        host[i] = SceneDev{a, (const float2 *)c.prof, c.ant, c.r0, c.f0, (float2 *)c.img};
+        const int nt = a.tiles_x * a.tiles_y;
+        for (int t = 0; t < nt; t++) items.push_back(Item{i, t, (double)c.Np * tx * ty});
+    }
+    // largest first; within a scene keep tile order (stable sort)
+    std::stable_sort(items.begin(), items.end(),
+                     [](const Item &p, const Item &q) { return p.cost > q.cost; });
+    std::vector<int2> hitems(items.size());
+    for (size_t i = 0; i < items.size(); i++) hitems[i] = make_int2(items[i].scene, items[i].tile);
+    const size_t bytes_sc = sizeof(SceneDev) * nscenes, bytes_it = sizeof(int2) * hitems.size();
+    void *buf = nullptr;
+    SARF_TRY(scratch_get(ctx, "scenes.desc", bytes_sc + bytes_it + 256, &buf));
+    SceneDev *dsc = (SceneDev *)buf;
+    int2 *ditems = (int2 *)(((uintptr_t)((char *)buf + bytes_sc) + 15) & ~(uintptr_t)15);
+    unsigned int *counter = (unsigned int *)(((uintptr_t)((char *)ditems + bytes_it) + 15) & ~(uintptr_t)15);
+    hipStream_t s = (hipStream_t)stream;
+    // pageable source: the copy completes before returning, so the host
+    // vectors may go out of scope
+    SARF_HIP(hipMemcpyAsync(dsc, host.data(), bytes_sc, hipMemcpyHostToDevice, s));
+    SARF_HIP(hipMemcpyAsync(ditems, hitems.data(), bytes_it, hipMemcpyHostToDevice, s));
+    SARF_HIP(hipStreamSynchronize(s));
+    const int cus = ctx->cu_count > 0 ? ctx->cu_count : 256;
+    const int nit = (int)hitems.size();
+    const int grid = std::min(nit, 4 * cus);
+    switch (tid) {
+        case 1: launch_scenes_t<32, 4>(grid, dsc, ditems, nit, counter, s); break;
+        case 2: launch_scenes_t<64, 4>(grid, dsc, ditems, nit, counter, s); break;
+        case 3: launch_scenes_t<64, 8>(grid, dsc, ditems, nit, counter, s); break;
+        case 4: launch_scenes_t<32, 8>(grid, dsc, ditems, nit, counter, s); break;
+        case 5: launch_scenes_t<16, 4>(grid, dsc, ditems, nit, counter, s); break;
+    }
+    SARF_HIP(hipGetLastError());
+    return SARF_OK;
+}
