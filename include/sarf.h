@@ -256,6 +256,15 @@ int sarf_synth_phase_history(sarf_ctx *ctx, int K, int Np, int S,
                              double df, void *ph, /* device, float2 K x Np */
                              void *stream);
 
+/* Same data model for many scatterers (distributed clutter): a type-1      */
+/* non-uniform FFT per pulse (8-tap exponential-of-semicircle spreading on  */
+/* a 2x oversampled grid), O(S + K log K) per pulse instead of O(S K);      */
+/* ~1e-6 relative error.  Needs K <= 8192.  Adds into ph.                   */
+int sarf_synth_phase_history_nufft(sarf_ctx *ctx, int K, int Np, int S,
+                                   const double *scat, const float *amp,
+                                   const double *ant, const double *r0, const double *f0,
+                                   double df, void *ph, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,8 @@ SIGNATURES = {
     "sarf_bp_entropy_grad": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sarf_pga": (_i, [_vp, C.c_void_p, _vp, _vp, _vp, _vp]),
     "sarf_bp_scenes": (_i, [_vp, _i, C.c_void_p, _i, _vp]),
+    "sarf_synth_phase_history_nufft": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _vp,
+                                            _vp]),
 }
 
 

@@ -241,7 +241,10 @@ class Context:
     # --------------------------------------------------------------- synth
     def synth_phase_history(self, ant: torch.Tensor, r0: torch.Tensor, f0: torch.Tensor,
                             df: float, K: int, scat: torch.Tensor, amp: torch.Tensor,
-                            out: torch.Tensor | None = None) -> torch.Tensor:
+                            out: torch.Tensor | None = None, method: str = "direct") -> torch.Tensor:
+        """Add the phase history of point scatterers to `out` (zeros if None).
+        method "direct" sums every scatterer exactly (FP64 phase); "nufft"
+        spreads them through a non-uniform FFT (~1e-6 relative, for clutter)."""
         Np = ant.shape[0]
         S = scat.shape[0]
         _need(scat, torch.float64, (S, 3), "scat")
@@ -249,7 +252,8 @@ class Context:
         if out is None:
             out = torch.zeros((Np, K), dtype=torch.complex64, device=ant.device)
         _need(out, torch.complex64, (Np, K), "out")
-        check(self.lib.sarf_synth_phase_history(self._h, K, Np, S, _ptr(scat), _ptr(amp),
-                                                _ptr(ant), _ptr(r0), _ptr(f0), float(df),
-                                                _ptr(out), self._stream()))
+        fn = (self.lib.sarf_synth_phase_history_nufft if method == "nufft"
+              else self.lib.sarf_synth_phase_history)
+        check(fn(self._h, K, Np, S, _ptr(scat), _ptr(amp), _ptr(ant), _ptr(r0), _ptr(f0),
+                 float(df), _ptr(out), self._stream()))
         return out

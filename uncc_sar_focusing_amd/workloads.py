@@ -38,9 +38,59 @@ def _dev(a, dev):
 
 
 def synth(ctx, geom: Geometry, tg: Targets, dev="cuda:0", out=None) -> torch.Tensor:
+    """Phase history of the scatterers `tg`; many scatterers (clutter) go
+    through the non-uniform-FFT synthesis, a few are summed exactly."""
     ant, r0, f0 = _dev(geom.ant, dev), _dev(geom.r0, dev), _dev(geom.f0, dev)
+    method = "nufft" if tg.pos.shape[0] > 1000 and geom.K <= 8192 else "direct"
     return ctx.synth_phase_history(ant, r0, f0, geom.df, geom.K, _dev(tg.pos, dev),
-                                   _dev(tg.amp.astype(np.complex64), dev), out=out)
+                                   _dev(tg.amp.astype(np.complex64), dev), out=out, method=method)
+
+
+@dataclasses.dataclass
+class ServingScene:
+    geom: Geometry
+    grid: ImageGrid
+    nfft: int
+    ph: torch.Tensor      # (Np, K) complex64, with the azimuth phase error applied
+    ant: torch.Tensor
+    r0: torch.Tensor
+    f0: torch.Tensor
+    phase_err: np.ndarray  # (Np,) applied error (rad)
+
+    @property
+    def pixel_pulses(self) -> int:
+        return self.grid.Nx * self.grid.Ny * self.geom.Np
+
+
+def uneven_scenes(ctx, n: int = 64, seed: int = 4321, spacing: float = 0.05,
+                  dev: str = "cuda:0") -> list:
+    """n independent scenes: Np ~ U{256..4096}, K ~ U{256..2048}, image side
+    ~ U{128..1024} at `spacing`, 10-50 point targets, and an 8th-order
+    polynomial azimuth phase error sum_i c_i t^i (c_i ~ N(0, pi^2), t over
+    [-1, 1]) multiplied onto each pulse.  Nfft = 4 * nextpow2(K), <= 16384."""
+    from .scenes import normal
+
+    u = uniform(seed, 31, 5 * n).reshape(n, 5)
+    out = []
+    for i in range(n):
+        Np = 256 + int(u[i, 0] * (4096 - 256 + 1))
+        K = 256 + int(u[i, 1] * (2048 - 256 + 1))
+        side = 128 + int(u[i, 2] * (1024 - 128 + 1))
+        ntg = 10 + int(u[i, 3] * 41)
+        nfft = min(16384, 4 * (1 << int(np.ceil(np.log2(K)))))
+        geom = circular_aperture(Np, K)
+        grid = ImageGrid.square(side, spacing)
+        from .scenes import point_targets
+        tg = point_targets(ntg, 0.8 * spacing * (side - 1), seed + 97 * i, stream=5)
+        ph = synth(ctx, geom, tg, dev)
+        c = normal(seed + 13 * i, 41, 9) * np.pi
+        t = np.linspace(-1.0, 1.0, Np)
+        err = np.polyval(c[::-1], t)
+        ph *= torch.from_numpy(np.exp(1j * err).astype(np.complex64)).to(dev)[:, None]
+        out.append(ServingScene(geom, grid, nfft, ph, _dev(geom.ant, dev), _dev(geom.r0, dev),
+                                _dev(geom.f0, dev), err))
+    torch.cuda.synchronize()
+    return out
 
 
 def smoothed_random_walk(Np: int, sigma: float, seed: int, stream: int = 21,
