@@ -62,28 +62,38 @@ class ServingScene:
         return self.grid.Nx * self.grid.Ny * self.geom.Np
 
 
-def uneven_scenes(ctx, n: int = 64, seed: int = 4321, spacing: float = 0.05,
-                  dev: str = "cuda:0") -> list:
-    """n independent scenes: Np ~ U{256..4096}, K ~ U{256..2048}, image side
-    ~ U{128..1024} at `spacing`, 10-50 point targets, and an 8th-order
-    polynomial azimuth phase error sum_i c_i t^i (c_i ~ N(0, pi^2), t over
-    [-1, 1]) multiplied onto each pulse.  Nfft = 4 * nextpow2(K), <= 16384."""
+def uneven_scene_specs(n: int = 64, seed: int = 4321) -> list:
+    """(Np, K, side, n_targets, nfft, phase-error coefficients) of n scenes:
+    Np ~ U{256..4096}, K ~ U{256..2048}, side ~ U{128..1024}, 10-50 targets,
+    coefficients c_0..c_8 ~ N(0, pi^2); Nfft = 4 * nextpow2(K), <= 16384."""
     from .scenes import normal
 
     u = uniform(seed, 31, 5 * n).reshape(n, 5)
-    out = []
+    specs = []
     for i in range(n):
         Np = 256 + int(u[i, 0] * (4096 - 256 + 1))
         K = 256 + int(u[i, 1] * (2048 - 256 + 1))
         side = 128 + int(u[i, 2] * (1024 - 128 + 1))
         ntg = 10 + int(u[i, 3] * 41)
         nfft = min(16384, 4 * (1 << int(np.ceil(np.log2(K)))))
+        c = normal(seed + 13 * i, 41, 9) * np.pi
+        specs.append((Np, K, side, ntg, nfft, c))
+    return specs
+
+
+def uneven_scenes(ctx, n: int = 64, seed: int = 4321, spacing: float = 0.05,
+                  dev: str = "cuda:0") -> list:
+    """n independent scenes (uneven_scene_specs) at `spacing`, each with its
+    8th-order polynomial azimuth phase error sum_i c_i t^i (t over [-1, 1])
+    multiplied onto every pulse."""
+    from .scenes import circular_aperture, point_targets
+
+    out = []
+    for i, (Np, K, side, ntg, nfft, c) in enumerate(uneven_scene_specs(n, seed)):
         geom = circular_aperture(Np, K)
         grid = ImageGrid.square(side, spacing)
-        from .scenes import point_targets
         tg = point_targets(ntg, 0.8 * spacing * (side - 1), seed + 97 * i, stream=5)
         ph = synth(ctx, geom, tg, dev)
-        c = normal(seed + 13 * i, 41, 9) * np.pi
         t = np.linspace(-1.0, 1.0, Np)
         err = np.polyval(c[::-1], t)
         ph *= torch.from_numpy(np.exp(1j * err).astype(np.complex64)).to(dev)[:, None]
