@@ -129,6 +129,60 @@ int sarf_backproject(sarf_ctx *ctx, const sarf_bp_desc *desc,
                      void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Matched-filter image formation (exact, O(K) per pixel and pulse):        */
+/*   img[p] += sum_n sum_k ph[n*K + k] exp(j 4 pi f_{n,k} dR_n(p) / c),     */
+/*   f_{n,k} = f0[n] + k df[n],  dR_n(p) = |a_n - p| - R0_n,  z = 0,        */
+/* every pixel and every pulse (no swath gate).  Replaces run_mf +          */
+/* computeDifferentialRangeAndPhaseCorrectionsMF                            */
+/* (src/cpuBackProjection/cpuBackProjection.hpp:165-228) and                */
+/* matlab/mfBasic.m:66-89.  run_mf starts at pulse 2 (:218); callers that   */
+/* want that pass the pulses from index 2 on.  Range and phase steps per    */
+/* pixel and pulse in FP64; the sum over k in FP32 by Horner (K <= 8192).   */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int Nx, Ny;       /* image size, >= 2 each */
+    int K;            /* samples per pulse, 1..8192 */
+    int Np;           /* pulses, >= 1 */
+    double x0, y0, Wx, Wy;
+    int flags;        /* SARF_BP_ACCUMULATE */
+    int pulse_split;  /* 0 = automatic; n >= 1 as in sarf_bp_desc */
+} sarf_mf_desc;
+
+int sarf_matched_filter(sarf_ctx *ctx, const sarf_mf_desc *desc,
+                        const void *ph,     /* device, complex64 K x Np (ph[n*K + k]) */
+                        const double *ant,  /* device, Np x 3 */
+                        const double *r0,   /* device, Np */
+                        const double *f0,   /* device, Np: frequency of sample 0 */
+                        const double *df,   /* device, Np: frequency step */
+                        void *img,          /* device, complex64 Ny x Nx */
+                        void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Launch-shape tuner for sarf_backproject (mixed precision).  Times every   */
+/* pixel tile that meets the same accuracy bound as the default choice,     */
+/* crossed with pulse splits 1, 2, 4, .. 32 (>= 32 pulses per split), on the */
+/* caller's buffers and stream, and waits for the stream.  The fastest pair  */
+/* is remembered in the context for this image geometry (Nx, Ny, Np, Nfft,  */
+/* spacing, maxWr, LOS hint): later sarf_backproject calls with tile = 0 and */
+/* pulse_split = 0 use it.  On return img holds the backprojection made     */
+/* with the chosen pair.  desc->flags must not hold SARF_BP_ACCUMULATE;     */
+/* desc->tile and desc->pulse_split are ignored.  No reference counterpart: */
+/* the reference launches one fixed block shape (src/gpuBackProjection/     */
+/* cuda_sar_focusing/cuda_sar_focusing.hpp:632-634).                        */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int tile;          /* chosen tile id */
+    int pulse_split;   /* chosen split count */
+    float ms;          /* its mean time per call (ms) */
+    float default_ms;  /* mean time of the untuned default choice (ms) */
+    int candidates;    /* (tile, split) pairs timed */
+} sarf_tune_result;
+
+int sarf_autotune(sarf_ctx *ctx, const sarf_bp_desc *desc, const void *prof, const double *ant,
+                  const double *r0, const double *f0, void *img, int reps,
+                  sarf_tune_result *out, void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Image entropy E = -sum p ln p, p = |I|^2 / sum |I|^2 (FP64 sums).         */
 /* out (device, 3 doubles) = {E, S = sum |I|^2, T = sum |I|^2 ln |I|^2}.     */
 /* The focus metric of every autofocus path here; it replaces the           */

@@ -277,3 +277,71 @@ def test_synth_nufft_matches_direct_oracle(ctx, K, Np, S):
                                  to_dev(amp.astype(np.complex64)), method="nufft").cpu().numpy()
     ref = oracle.synth_ph_f64(geom, tg)
     assert oracle.rel_l2(ph, ref) < 2e-5, oracle.rel_l2(ph, ref)
+
+
+def test_autotune_chooses_an_accurate_shape_and_is_used_by_default():
+    """sarf_autotune on C0: the chosen (tile, split) image meets the same
+    1e-4 bound as the default; a later default-shaped call on the same
+    geometry reuses it (bit-identical to the explicit pair); other
+    geometries are unaffected."""
+    from uncc_sar_focusing_amd.api import Context
+
+    tctx = Context(0)  # own context: the tuned shape is per context
+    try:
+        cfg, geom, tg, grid, ph = c0_inputs()
+        ant, r0, f0 = geom_dev(geom)
+        prof = tctx.range_compress(to_dev(ph), cfg.nfft)
+        res, img = tctx.autotune(prof, ant, r0, f0, grid, geom.maxWr, reps=2)
+        assert 1 <= res["tile"] <= 7 and res["pulse_split"] >= 1
+        assert res["candidates"] >= 2
+        assert 0 < res["ms"] <= res["default_ms"] * 1.0001
+        ref = oracle.bp_f64(oracle.range_compress_f64(ph, cfg.nfft), geom.ant, geom.r0, geom.f0,
+                            grid, geom.maxWr)
+        img = img.cpu().numpy()
+        assert oracle.rel_l2(img, ref) <= 1e-4
+        assert local_peaks(img, grid, tg.pos) == local_peaks(ref, grid, tg.pos)
+        dflt = tctx.backproject(prof, ant, r0, f0, grid, geom.maxWr).cpu()
+        expl = tctx.backproject(prof, ant, r0, f0, grid, geom.maxWr, tile=res["tile"],
+                                pulse_split=res["pulse_split"]).cpu()
+        assert torch.equal(dflt, expl)
+        assert torch.equal(dflt, torch.from_numpy(img))
+        with pytest.raises(_lib.SarfError) as e:
+            tctx.autotune(prof, ant, r0, f0, grid, geom.maxWr, reps=0)
+        assert e.value.code == _lib.SARF_ERR_INVALID
+    finally:
+        tctx.close()
+
+
+# ------------------------------------------------------------ matched filter
+@pytest.mark.parametrize("Np,K,Nx,Ny,split", [(24, 96, 40, 36, 0), (8, 1501, 20, 70, 0),
+                                              (33, 64, 17, 130, 5), (3, 1, 5, 3, 0)])
+def test_matched_filter_matches_fp64_oracle(ctx, Np, K, Nx, Ny, split):
+    """Exact matched filter (run_mf / mfBasic.m) on the GPU vs the FP64
+    oracle, including K over one LDS stage with an odd tail, ragged tiles,
+    explicit pulse splits and K = 1.  Tolerance: rel-L2 <= 1e-4."""
+    geom, tg, ph = small_inputs(Np=Np, K=K, extent=2.0)
+    grid = ImageGrid(Nx, Ny, 0.2, -0.1, 3.0, 2.5)
+    ant, r0, f0 = geom_dev(geom)
+    img = ctx.matched_filter(to_dev(ph), ant, r0, f0, geom.df, grid, pulse_split=split)
+    ref = oracle.mf_f64(ph, geom.ant, geom.r0, geom.f0, geom.df, grid)
+    e = oracle.rel_l2(img.cpu().numpy(), ref)
+    print(f"MF Np={Np} K={K}: rel-L2 {e:.2e}")
+    assert e <= 1e-4
+
+
+def test_matched_filter_accumulate_and_limits(ctx):
+    geom, tg, ph = small_inputs(Np=20, K=64, extent=2.0)
+    grid = ImageGrid.square(24, 0.1)
+    ant, r0, f0 = geom_dev(geom)
+    phd = to_dev(ph)
+    full = ctx.matched_filter(phd, ant, r0, f0, geom.df, grid)
+    part = ctx.matched_filter(phd[:7].contiguous(), ant[:7].contiguous(), r0[:7].contiguous(),
+                              f0[:7].contiguous(), geom.df, grid)
+    ctx.matched_filter(phd[7:].contiguous(), ant[7:].contiguous(), r0[7:].contiguous(),
+                       f0[7:].contiguous(), geom.df, grid, out=part, accumulate=True)
+    assert oracle.rel_l2(part.cpu().numpy(), full.cpu().numpy()) < 1e-6
+    big = torch.zeros((1, 8193), dtype=torch.complex64, device=DEV)
+    with pytest.raises(_lib.SarfError) as e:
+        ctx.matched_filter(big, ant[:1].contiguous(), r0[:1].contiguous(), f0[:1].contiguous(),
+                           geom.df, grid)
+    assert e.value.code == _lib.SARF_ERR_UNSUPPORTED

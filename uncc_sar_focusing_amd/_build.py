@@ -59,6 +59,18 @@ def build_library(verbose: bool = True) -> str:
     return LIB_PATH
 
 
+PROBE_PATH = os.path.join(PKG_DIR, "libsarf_probe.so")
+
+
+def build_probe(verbose: bool = True) -> str:
+    """Compile the peak-rate microbenchmarks (probe/) into libsarf_probe.so."""
+    src = os.path.join(PKG_DIR, "probe", "sarf_probe.hip")
+    tmp = PROBE_PATH + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, src], verbose)
+    os.replace(tmp, PROBE_PATH)
+    return PROBE_PATH
+
+
 def build_examples(verbose: bool = True) -> list[str]:
     """Compile the C++ host examples against include/ and libsarf.so."""
     out = []
@@ -85,6 +97,7 @@ def build_oracle(verbose: bool = True) -> str | None:
 
 def main() -> int:
     build_library(True)
+    build_probe(True)
     build_examples(True)
     build_oracle(True)
     return 0

@@ -78,7 +78,22 @@ SIGNATURES = {
     "sarf_bp_scenes": (_i, [_vp, _i, C.c_void_p, _i, _vp]),
     "sarf_synth_phase_history_nufft": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _vp,
                                             _vp]),
+    "sarf_autotune": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp]),
+    "sarf_matched_filter": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
+
+
+class MfDesc(C.Structure):
+    _fields_ = [
+        ("Nx", C.c_int), ("Ny", C.c_int), ("K", C.c_int), ("Np", C.c_int),
+        ("x0", C.c_double), ("y0", C.c_double), ("Wx", C.c_double), ("Wy", C.c_double),
+        ("flags", C.c_int), ("pulse_split", C.c_int),
+    ]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("tile", C.c_int), ("pulse_split", C.c_int), ("ms", C.c_float),
+                ("default_ms", C.c_float), ("candidates", C.c_int)]
 
 
 class SceneDesc(C.Structure):

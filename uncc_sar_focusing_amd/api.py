@@ -118,6 +118,55 @@ class Context:
                                         _ptr(f0), _ptr(out), self._stream()))
         return out
 
+    def matched_filter(self, ph: torch.Tensor, ant: torch.Tensor, r0: torch.Tensor,
+                       f0: torch.Tensor, df: torch.Tensor | float, grid: ImageGrid, *,
+                       out: torch.Tensor | None = None, accumulate: bool = False,
+                       pulse_split: int = 0) -> torch.Tensor:
+        """Exact matched-filter image (run_mf, cpuBackProjection.hpp:165-228):
+        sum over pulses and samples of ph * exp(j 4 pi (f0 + k df) dR / c).
+        `df` is the per-pulse frequency step (tensor (Np,)) or one value."""
+        Np, K = ph.shape
+        _need(ph, torch.complex64, (Np, K), "ph")
+        _need(ant, torch.float64, (Np, 3), "ant")
+        _need(r0, torch.float64, (Np,), "r0")
+        _need(f0, torch.float64, (Np,), "f0")
+        if not isinstance(df, torch.Tensor):
+            df = torch.full((Np,), float(df), dtype=torch.float64, device=ph.device)
+        _need(df, torch.float64, (Np,), "df")
+        if out is None:
+            alloc = torch.zeros if accumulate else torch.empty
+            out = alloc((grid.Nx, grid.Ny), dtype=torch.complex64, device=ph.device)
+        _need(out, torch.complex64, (grid.Nx, grid.Ny), "out")
+        d = _lib.MfDesc(grid.Nx, grid.Ny, K, Np, grid.x0, grid.y0, grid.Wx, grid.Wy,
+                        _lib.SARF_BP_ACCUMULATE if accumulate else 0, pulse_split)
+        check(self.lib.sarf_matched_filter(self._h, C.byref(d), _ptr(ph), _ptr(ant), _ptr(r0),
+                                           _ptr(f0), _ptr(df), _ptr(out), self._stream()))
+        return out
+
+    def autotune(self, prof: torch.Tensor, ant: torch.Tensor, r0: torch.Tensor,
+                 f0: torch.Tensor, grid: ImageGrid, maxWr: float, *, reps: int = 3,
+                 out: torch.Tensor | None = None,
+                 los: tuple[float, float] | None = None) -> tuple[dict, torch.Tensor]:
+        """Time every accurate (tile, pulse split) pair for this geometry on
+        these inputs; later `backproject` calls with tile=0, pulse_split=0
+        and the same geometry use the fastest.  Returns ({tile, pulse_split,
+        ms, default_ms, candidates}, image made with the chosen pair)."""
+        Np, nfft = prof.shape
+        _need(prof, torch.complex64, (Np, nfft), "prof")
+        _need(ant, torch.float64, (Np, 3), "ant")
+        _need(r0, torch.float64, (Np,), "r0")
+        _need(f0, torch.float64, (Np,), "f0")
+        if out is None:
+            out = torch.empty((grid.Nx, grid.Ny), dtype=torch.complex64, device=prof.device)
+        _need(out, torch.complex64, (grid.Nx, grid.Ny), "out")
+        d = BpDesc(grid.Nx, grid.Ny, nfft, Np, grid.x0, grid.y0, grid.Wx, grid.Wy, maxWr,
+                   _lib.SARF_BP_MIXED, 0, 0, 0, *(los if los is not None else (0.0, 0.0)))
+        r = _lib.TuneResult()
+        check(self.lib.sarf_autotune(self._h, C.byref(d), _ptr(prof), _ptr(ant), _ptr(r0),
+                                     _ptr(f0), _ptr(out), reps, C.byref(r), self._stream()))
+        res = {k: getattr(r, k) for k, _ in _lib.TuneResult._fields_}
+        return res, out
+
     # ------------------------------------------------------------- entropy
     def image_entropy(self, img: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """Device tensor [E, S, T] (float64): E = -sum p ln p, p = |I|^2/S."""

@@ -7,6 +7,8 @@
 // the "all-FP64 build" the mixed kernel is measured against at full size.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <vector>
 
 #include "sarf_bp_core.hpp"
 
@@ -210,35 +212,40 @@ int bp_fill_args(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, BpArgs *out) {
     return SARF_OK;
 }
 
-int launch_backproject(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof, const double *ant,
-                       const double *r0, const double *f0, void *img, hipStream_t s) {
-    if (d->precision != SARF_BP_MIXED && d->precision != SARF_BP_F64)
-        return fail(SARF_ERR_INVALID, "backproject: unknown precision %d", d->precision);
-    const int tile = d->precision == SARF_BP_MIXED
-                         ? (d->tile > 0 ? d->tile : 0)
-                         : 0;
+namespace {
+
+std::array<uint64_t, 8> tune_key(const sarf_bp_desc *d) {
+    auto bits = [](double v) {
+        uint64_t u;
+        std::memcpy(&u, &v, sizeof u);
+        return u;
+    };
+    return {(uint64_t)d->Nx, (uint64_t)d->Ny, (uint64_t)d->Np, (uint64_t)d->Nfft,
+            bits(d->Wx / (d->Nx - 1)), bits(d->Wy / (d->Ny - 1)), bits(d->maxWr),
+            bits(d->los_x) * 31 + bits(d->los_y)};
+}
+
+// Default pulse splits: with fewer than 4 tiles per CU, split the pulses
+// into groups that form partial images (summed in order afterwards),
+// keeping at least 64 pulses per group.
+int default_splits(const sarf_ctx *ctx, int ntiles, int Np) {
+    int splits = 1;
+    const int want = 4 * (ctx->cu_count > 0 ? ctx->cu_count : 256);
+    while (splits < 32 && ntiles * splits < want && Np / (2 * splits) >= 64) splits *= 2;
+    return splits;
+}
+
+// Mixed-precision launch with a resolved tile id (> 0) and split count
+// (0 = default rule).
+int launch_mixed_shape(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, int splits,
+                       const void *prof, const double *ant, const double *r0, const double *f0,
+                       void *img, hipStream_t s) {
     BpArgs a{};
     SARF_TRY(bp_fill_args(ctx, d, tile, &a));
-    if (d->precision == SARF_BP_F64) {
-        const long npix = (long)d->Nx * d->Ny;
-        hipLaunchKernelGGL(bp_f64, dim3((npix + 255) / 256), dim3(256), 0, s, a,
-                           (const double2 *)prof, ant, r0, f0, (double2 *)img);
-        SARF_HIP(hipGetLastError());
-        return SARF_OK;
-    }
     int tx = 0, ty = 0;
-    const int tid = bp_tile_shape(tile > 0 ? tile : bp_pick_tile(d, a.dx, a.dy), &tx, &ty);
+    const int tid = bp_tile_shape(tile, &tx, &ty);
     const int ntiles = a.tiles_x * a.tiles_y;
-    // Fill the GPU: with fewer than 4 tiles per CU, split the pulses into
-    // groups that form partial images (summed in order afterwards), keeping
-    // at least 64 pulses per group.
-    int splits = 1;
-    if (d->pulse_split > 0) {
-        splits = d->pulse_split;  // explicit (tests, autotuner)
-    } else {
-        const int want = 4 * (ctx->cu_count > 0 ? ctx->cu_count : 256);
-        while (splits < 32 && ntiles * splits < want && d->Np / (2 * splits) >= 64) splits *= 2;
-    }
+    if (splits <= 0) splits = default_splits(ctx, ntiles, d->Np);
     splits = std::max(1, std::min(splits, d->Np));
     const long npix = (long)d->Nx * d->Ny;
     float2 *o = (float2 *)img;
@@ -275,4 +282,152 @@ int launch_backproject(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof, c
     return SARF_OK;
 }
 
+}  // namespace
+
+int launch_reduce_partials(const float2 *part, long stride, int splits, long n, int accumulate,
+                           float2 *img, hipStream_t s) {
+    hipLaunchKernelGGL(bp_reduce_splits, dim3((n + 255) / 256), dim3(256), 0, s, part, stride,
+                       splits, n, accumulate, img);
+    SARF_HIP(hipGetLastError());
+    return SARF_OK;
+}
+
+int launch_backproject(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof, const double *ant,
+                       const double *r0, const double *f0, void *img, hipStream_t s) {
+    if (d->precision != SARF_BP_MIXED && d->precision != SARF_BP_F64)
+        return fail(SARF_ERR_INVALID, "backproject: unknown precision %d", d->precision);
+    if (d->precision == SARF_BP_F64) {
+        BpArgs a{};
+        SARF_TRY(bp_fill_args(ctx, d, 0, &a));
+        const long npix = (long)d->Nx * d->Ny;
+        hipLaunchKernelGGL(bp_f64, dim3((npix + 255) / 256), dim3(256), 0, s, a,
+                           (const double2 *)prof, ant, r0, f0, (double2 *)img);
+        SARF_HIP(hipGetLastError());
+        return SARF_OK;
+    }
+    int tile = d->tile > 0 ? d->tile : 0;
+    int splits = d->pulse_split > 0 ? d->pulse_split : 0;
+    if (tile == 0 && splits == 0 && d->Nx >= 2 && d->Ny >= 2) {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        auto it = ctx->tuned.find(tune_key(d));
+        if (it != ctx->tuned.end()) {
+            tile = it->second.first;
+            splits = it->second.second;
+        }
+    }
+    if (tile == 0) {
+        if (d->Nx < 2 || d->Ny < 2 || !(d->Wx > 0) || !(d->Wy > 0))
+            return fail(SARF_ERR_INVALID, "backproject: bad sizes Nx=%d Ny=%d Wx=%g Wy=%g", d->Nx,
+                        d->Ny, d->Wx, d->Wy);
+        tile = bp_pick_tile(d, d->Wx / (d->Nx - 1), d->Wy / (d->Ny - 1));
+    }
+    return launch_mixed_shape(ctx, d, tile, splits, prof, ant, r0, f0, img, s);
+}
+
 }  // namespace sarf
+
+using namespace sarf;
+
+namespace {
+
+// Tiles that meet the default choice's accuracy bound (and the LDS window).
+std::vector<int> tune_tiles(const sarf_bp_desc *d) {
+    const double dx = d->Wx / (d->Nx - 1), dy = d->Wy / (d->Ny - 1);
+    const double invbin = d->Nfft / d->maxWr;
+    const bool hint = (d->los_x != 0.0 || d->los_y != 0.0);
+    const double lx = hint ? std::fabs(d->los_x) : 1.0, ly = hint ? std::fabs(d->los_y) : 1.0;
+    const double bound = hint ? 2.6 : 3.2;
+    std::vector<int> out;
+    for (int id = 1; id <= bp_tile_count(); id++) {
+        int tx = 0, ty = 0;
+        if (!bp_tile_shape(id, &tx, &ty)) continue;
+        const double hx = 0.5 * (tx - 1) * dx, hy = 0.5 * (ty - 1) * dy;
+        if (lx * hx + ly * hy > bound) continue;
+        if (2.0 * (hx + hy) * invbin + 8 > kBpCap) continue;
+        out.push_back(id);
+    }
+    return out;
+}
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+}  // namespace
+
+extern "C" int sarf_autotune(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof,
+                             const double *ant, const double *r0, const double *f0, void *img,
+                             int reps, sarf_tune_result *out, void *stream) {
+    clear_error();
+    SARF_TRY(use_device(ctx));
+    if (!d || !prof || !ant || !r0 || !f0 || !img || !out)
+        return fail(SARF_ERR_INVALID, "sarf_autotune: null argument");
+    if (d->precision != SARF_BP_MIXED)
+        return fail(SARF_ERR_INVALID, "sarf_autotune: only SARF_BP_MIXED has launch shapes");
+    if (d->flags & SARF_BP_ACCUMULATE)
+        return fail(SARF_ERR_INVALID, "sarf_autotune: SARF_BP_ACCUMULATE is not allowed");
+    if (reps < 1 || reps > 100) return fail(SARF_ERR_INVALID, "sarf_autotune: reps=%d", reps);
+    BpArgs chk{};
+    SARF_TRY(bp_fill_args(ctx, d, 0, &chk));  // validates sizes and geometry
+    hipStream_t s = (hipStream_t)stream;
+    sarf_bp_desc dd = *d;
+    dd.tile = 0;
+    dd.pulse_split = 0;
+    EventPair ev;
+    SARF_HIP(hipEventCreate(&ev.a));
+    SARF_HIP(hipEventCreate(&ev.b));
+    auto time_shape = [&](int tile, int splits, float *ms) -> int {
+        SARF_TRY(launch_mixed_shape(ctx, &dd, tile, splits, prof, ant, r0, f0, img, s));  // warm
+        SARF_HIP(hipEventRecord(ev.a, s));
+        for (int r = 0; r < reps; r++)
+            SARF_TRY(launch_mixed_shape(ctx, &dd, tile, splits, prof, ant, r0, f0, img, s));
+        SARF_HIP(hipEventRecord(ev.b, s));
+        SARF_HIP(hipEventSynchronize(ev.b));
+        float t = 0.f;
+        SARF_HIP(hipEventElapsedTime(&t, ev.a, ev.b));
+        *ms = t / reps;
+        return SARF_OK;
+    };
+    const double dx = d->Wx / (d->Nx - 1), dy = d->Wy / (d->Ny - 1);
+    const int def_tile = bp_pick_tile(d, dx, dy);
+    float def_ms = 0.f;
+    SARF_TRY(time_shape(def_tile, 0, &def_ms));
+    int best_tile = def_tile, best_split = 0, n = 1;
+    float best_ms = def_ms;
+    for (int tile : tune_tiles(d)) {
+        for (int splits = 1; splits <= 32; splits *= 2) {
+            if (splits > 1 && d->Np / splits < 32) break;
+            float ms = 0.f;
+            SARF_TRY(time_shape(tile, splits, &ms));
+            n++;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_tile = tile;
+                best_split = splits;
+            }
+        }
+    }
+    if (best_split == 0) {  // the default won: record its resolved split count
+        int tx = 0, ty = 0;
+        bp_tile_shape(best_tile, &tx, &ty);
+        const int ntiles = ((d->Nx + tx - 1) / tx) * ((d->Ny + ty - 1) / ty);
+        best_split = std::max(1, std::min(default_splits(ctx, ntiles, d->Np), d->Np));
+    }
+    // leave the image made with the chosen shape
+    SARF_TRY(launch_mixed_shape(ctx, &dd, best_tile, best_split, prof, ant, r0, f0, img, s));
+    SARF_HIP(hipStreamSynchronize(s));
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->tuned[tune_key(d)] = {best_tile, best_split};
+    }
+    out->tile = best_tile;
+    out->pulse_split = best_split;
+    out->ms = best_ms;
+    out->default_ms = def_ms;
+    out->candidates = n;
+    return SARF_OK;
+}

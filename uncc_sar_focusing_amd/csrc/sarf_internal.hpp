@@ -5,6 +5,7 @@
 
 #include <cstdarg>
 #include <cstddef>
+#include <array>
 #include <cstdint>
 #include <map>
 #include <mutex>
@@ -53,6 +54,9 @@ struct sarf_ctx {
     std::map<long, double2 *> tw64;
     // named scratch buffers (partial images, reductions, staging)
     std::map<std::string, sarf::Scratch> scratch;
+    // backprojection launch shapes chosen by sarf_autotune, keyed by the
+    // image geometry: (tile id, pulse splits)
+    std::map<std::array<uint64_t, 8>, std::pair<int, int>> tuned;
 };
 
 namespace sarf {
@@ -76,6 +80,9 @@ int launch_range_compress(sarf_ctx *ctx, const sarf_rc_desc *d, const void *ph,
 int launch_backproject(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof,
                        const double *ant, const double *r0, const double *f0, void *img,
                        hipStream_t stream);
+// img[i] (+)= sum over s in order of part[s*stride + i], i < n (deterministic).
+int launch_reduce_partials(const float2 *part, long stride, int splits, long n, int accumulate,
+                           float2 *img, hipStream_t stream);
 
 }  // namespace sarf
 
