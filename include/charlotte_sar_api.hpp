@@ -49,46 +49,56 @@ using NumericType = float;
 using ComplexType = Complex<NumericType>;
 using ComplexArrayType = CArray<NumericType>;
 
-// Magnitude image in dB over [-dyn_range_dB, 0] mapped to 0..255, written as
-// a 32-bit BMP with far range at the top (uncc_sar_globals.hpp:66-118).
+// Display image and file in the reference's format (uncc_sar_globals.hpp:
+// 66-118 with loadbmp.h:182-245): |I| in dB over [-dyn_range_dB, 0] mapped to
+// 0..255 in float, far range (largest y) in the top row, written as a
+// 24-bit grey BMP, rows bottom-up and padded to 4 bytes, header size field
+// 54 + 3*W*H.
 template <typename T, typename P>
 int writeBMPFile(const SAR_ImageFormationParameters<P> &params, const CArray<T> &img,
                  const std::string &filename) {
     const int W = params.N_x_pix, H = params.N_y_pix;
     if ((long)img.size() < (long)W * H || W <= 0 || H <= 0) return EXIT_FAILURE;
-    double vmax = 0.0;
-    for (size_t i = 0; i < img.size(); i++) vmax = std::max(vmax, (double)Complex<T>::abs(img[i]));
-    const double dyn = params.dyn_range_dB;
-    const uint32_t row = 4u * W, bytes = row * H, off = 54;
-    std::vector<unsigned char> buf(off + bytes, 0);
-    auto put32 = [&](size_t at, uint32_t v) {
-        for (int b = 0; b < 4; b++) buf[at + b] = (unsigned char)(v >> (8 * b));
-    };
-    buf[0] = 'B';
-    buf[1] = 'M';
-    put32(2, off + bytes);
-    put32(10, off);
-    put32(14, 40);
-    put32(18, (uint32_t)W);
-    put32(22, (uint32_t)H);  // positive height: rows stored bottom-up
-    buf[26] = 1;
-    buf[28] = 32;
-    put32(34, bytes);
-    for (int r = 0; r < H; r++) {      // display row r (0 = top = far range)
-        const int y = H - 1 - r;       // image row index (y fastest in img)
-        const int stored = H - 1 - r;  // bottom-up storage
+    float vmax = 0.0f;
+    for (size_t i = 0; i < img.size(); i++) vmax = std::max(vmax, (float)Complex<T>::abs(img[i]));
+    const float dyn = (float)params.dyn_range_dB;
+    const float scale = (float)(255.0 / dyn);
+    // top-down display rows: row r shows y = H-1-r
+    std::vector<unsigned char> rows((size_t)W * H);
+    for (int r = 0; r < H; r++) {
+        const int y = H - 1 - r;
         for (int x = 0; x < W; x++) {
-            const double a = Complex<T>::abs(img[(size_t)x * H + y]);
-            double v = vmax > 0 ? 255.0 / dyn * (20.0 * std::log10(a / vmax) + dyn) : 0.0;
-            const unsigned char px = (v < 0 || !(v == v)) ? 0 : (unsigned char)std::min(v, 255.0);
-            unsigned char *q = &buf[off + (size_t)stored * row + 4 * x];
-            q[0] = q[1] = q[2] = px;
-            q[3] = 255;
+            const float a = (float)Complex<T>::abs(img[(size_t)x * H + y]);
+            const float l = std::log10(a / vmax);
+            volatile float t = 20.0f * l;  // no contraction: same rounding as the reference
+            volatile float u = t + dyn;
+            const float v = scale * u;
+            rows[(size_t)r * W + x] = (v < 0.0f || !(v == v)) ? 0 : (unsigned char)v;
         }
+    }
+    const uint32_t size = 54u + 3u * (uint32_t)W * (uint32_t)H;
+    unsigned char hdr[54] = {'B', 'M'};
+    auto put32 = [&](int at, uint32_t v) {
+        for (int b = 0; b < 4; b++) hdr[at + b] = (unsigned char)(v >> (8 * b));
+    };
+    put32(2, size);
+    hdr[10] = 54;
+    hdr[14] = 40;
+    put32(18, (uint32_t)W);
+    put32(22, (uint32_t)H);
+    hdr[26] = 1;
+    hdr[28] = 24;
+    const int pad = (4 - (W * 3) % 4) % 4;
+    std::vector<unsigned char> body;
+    body.reserve((size_t)(3 * W + pad) * H);
+    for (int r = H - 1; r >= 0; r--) {  // bottom-up
+        for (int x = 0; x < W; x++) body.insert(body.end(), 3, rows[(size_t)r * W + x]);
+        body.insert(body.end(), pad, 0);
     }
     std::ofstream f(filename, std::ios::binary);
     if (!f) return EXIT_FAILURE;
-    f.write((const char *)buf.data(), buf.size());
+    f.write((const char *)hdr, sizeof hdr);
+    f.write((const char *)body.data(), body.size());
     return f ? EXIT_SUCCESS : EXIT_FAILURE;
 }
 

@@ -319,6 +319,21 @@ int sarf_synth_phase_history_nufft(sarf_ctx *ctx, int K, int Np, int S,
                                    const double *ant, const double *r0, const double *f0,
                                    double df, void *ph, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* 8-bit dB display image: the quantisation of writeBMPFile                 */
+/* (include/uncc_sar_globals.hpp:66-118), in float as the reference does it: */
+/*   pixelf = (float)(255.0/dyn) * (20 log10(|I| / max|I|) + dyn),          */
+/*   byte = pixelf < 0 ? 0 : (unsigned char)pixelf,                         */
+/* rows top-down, row r showing y = Ny-1-r (far range on top, :85-98):      */
+/* out[r*Nx + x].  log10 is the device library's, so a byte can differ by   */
+/* one level from the host's where pixelf lands within an ulp of an integer.*/
+/* ------------------------------------------------------------------------ */
+int sarf_image_db8(sarf_ctx *ctx,
+                   const void *img,        /* device, complex64 Ny x Nx (img[x*Ny + y]) */
+                   int Nx, int Ny, float dyn_range_dB,
+                   unsigned char *out,     /* device, Nx*Ny bytes */
+                   void *stream);
+
 #ifdef __cplusplus
 }
 #endif

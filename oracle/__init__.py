@@ -49,6 +49,7 @@ _SIG = {
     "orc_synth_ph_f64": (_i, [_i, _i, _i, _P, _P, _P, _P, _P, _d, _P]),
     "orc_entropy_f64": (_d, [_P, C.c_long]),
     "orc_bp_adjoint_f64": (_i, [_i, _i, _i, _i, _d, _d, _d, _d, _d, _P, _P, _P, _P, _P, _P]),
+    "orc_db8": (_i, [_i, _i, _P, _f, _P]),
 }
 
 _LIB = None
@@ -181,6 +182,17 @@ def synth_ph_f64(geom, targets) -> np.ndarray:
     lib().orc_synth_ph_f64(K, Np, pos.shape[0], _p(pos), _p(amp), _p(ant), _p(r0), _p(f0),
                            geom.df, _p(ph))
     return ph
+
+
+def db8(img, dyn_range_db: float = 70.0) -> np.ndarray:
+    """8-bit dB display image (Ny, Nx), top row = far range, as the
+    reference's writeBMPFile quantises (uncc_sar_globals.hpp:66-118)."""
+    img = np.ascontiguousarray(img, dtype=np.complex64)
+    Nx, Ny = img.shape
+    out = np.empty((Ny, Nx), dtype=np.uint8)
+    if lib().orc_db8(Nx, Ny, _p(img), dyn_range_db, _p(out)) != 0:
+        raise ValueError("bad db8 arguments")
+    return out
 
 
 def entropy_f64(img) -> float:

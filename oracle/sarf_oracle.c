@@ -537,3 +537,45 @@ int orc_bp_adjoint_f64(int Nx, int Ny, int nfft, int Np, double x0, double y0,
     free(r);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* 8-bit dB display image, as the reference's writeBMPFile computes it      */
+/* (include/uncc_sar_globals.hpp:66-118), in float with no contraction:     */
+/*   m = max |I| (scaled abs of include/uncc_complex.hpp:93-102),           */
+/*   pixelf = (float)(255.0 / dyn) * (20 * log10f(|I| / m) + dyn),          */
+/*   byte = pixelf < 0 ? 0 : (unsigned char)pixelf,                         */
+/* rows top-down with row r showing y = Ny-1-r (the flipX branch, :85-98):  */
+/* out[r*Nx + x].  Parity: pinned by closed-form levels in test_oracle.py.  */
+/* ------------------------------------------------------------------------ */
+static float scaled_abs_f(float x, float y) {
+    float s = fmaxf(fabsf(x), fabsf(y));
+    if (s == 0.0f) return s;
+    x /= s;
+    y /= s;
+    float xx = x * x, yy = y * y;
+    return s * sqrtf(xx + yy);
+}
+
+int orc_db8(int Nx, int Ny, const float *img, float dyn, unsigned char *out) {
+    if (Nx < 1 || Ny < 1 || !(dyn > 0)) return -1;
+    long n = (long)Nx * Ny;
+    float m = 0.0f;
+    for (long i = 0; i < n; i++) {
+        float a = scaled_abs_f(img[2 * i], img[2 * i + 1]);
+        if (a > m) m = a;
+    }
+    const float scale = (float)(255.0 / (double)dyn);
+    for (int r = 0; r < Ny; r++) {
+        int y = Ny - 1 - r;
+        for (int x = 0; x < Nx; x++) {
+            long i = (long)x * Ny + y;
+            float a = scaled_abs_f(img[2 * i], img[2 * i + 1]);
+            float l = log10f(a / m);
+            float t = 20.0f * l;
+            float u = t + dyn;
+            float v = scale * u;
+            out[(long)r * Nx + x] = (v < 0.0f || v != v) ? 0 : (unsigned char)v;
+        }
+    }
+    return 0;
+}

@@ -345,3 +345,29 @@ def test_matched_filter_accumulate_and_limits(ctx):
         ctx.matched_filter(big, ant[:1].contiguous(), r0[:1].contiguous(), f0[:1].contiguous(),
                            geom.df, grid)
     assert e.value.code == _lib.SARF_ERR_UNSUPPORTED
+
+
+# --------------------------------------------------------------- dB output
+@pytest.mark.parametrize("Nx,Ny", [(256, 256), (70, 45), (1, 130)])
+def test_image_db8_matches_oracle(ctx, Nx, Ny):
+    """GPU 8-bit dB display image vs the oracle's restatement of the
+    reference's float quantisation: the device log10 may put a value within
+    an ulp of an integer on the other side, so at most 0.1 % of the bytes
+    may differ, by one level; row order and the max are exact."""
+    geom, tg, ph = small_inputs()
+    grid = ImageGrid(Nx, Ny, 0.1, -0.2, 0.05 * max(Nx - 1, 1), 0.05 * (Ny - 1)) if Nx > 1 else None
+    if grid is None:
+        rng = np.random.default_rng(9)
+        img = to_dev((rng.standard_normal((Nx, Ny)) + 1j * rng.standard_normal((Nx, Ny)))
+                     .astype(np.complex64))
+    else:
+        ant, r0, f0 = geom_dev(geom)
+        prof = ctx.range_compress(to_dev(ph), 1024)
+        img = ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr)
+    got = ctx.image_db8(img, 70.0).cpu().numpy()
+    ref = oracle.db8(img.cpu().numpy(), 70.0)
+    assert got.shape == ref.shape == (Ny, Nx)
+    d = np.abs(got.astype(int) - ref.astype(int))
+    assert d.max() <= 1
+    assert (d > 0).mean() <= 1e-3
+    assert got.max() == ref.max() == 255

@@ -320,3 +320,26 @@ def test_pga_recovers_injected_polynomial_phase_error():
     # up to a linear phase (a circular shift): entropy is shift invariant
     redo = np.fft.ifft(np.fft.fft(fixed, axis=0) * np.exp(1j * af_ph)[:, None], axis=0)
     assert abs(oracle.entropy_f64(redo) - e_bad) < 1e-6 * e_bad
+
+
+def test_db8_levels_and_row_order():
+    """dB display quantisation (uncc_sar_globals.hpp:66-118): closed-form
+    levels 255/70 (70 - d) truncated, zero and below-range pixels at 0,
+    far range (largest y) in the top row, x left to right, scale
+    invariance under powers of two."""
+    Nx, Ny = 5, 4
+    img = np.zeros((Nx, Ny), dtype=np.complex64)
+    img[0, 0] = 1.0                                  # 0 dB -> 255
+    img[1, 0] = 10 ** (-35 / 20) * 1j                # -35 dB -> 127.5 -> 127
+    img[2, 3] = 10 ** (-13 / 20) * (0.6 + 0.8j)      # -13 dB -> 207.6 -> 207
+    img[3, 2] = 10 ** (-80 / 20)                     # below the range -> 0
+    out = oracle.db8(img, 70.0)
+    assert out.shape == (Ny, Nx) and out.dtype == np.uint8
+    assert out[Ny - 1, 0] == 255 and out[Ny - 1, 1] == 127
+    assert out[0, 2] == 207
+    assert out[1, 3] == 0 and out[2, 4] == 0
+    assert int(out.astype(int).sum()) == 255 + 127 + 207
+    assert np.array_equal(oracle.db8(img * 8.0, 70.0), out)
+    assert np.array_equal(oracle.db8(img / 1024.0, 70.0), out)
+    # 40 dB dynamic range: -35 dB -> 255/40 * 5 = 31.875 -> 31
+    assert oracle.db8(img, 40.0)[Ny - 1, 1] == 31

@@ -143,6 +143,20 @@ class Context:
                                            _ptr(f0), _ptr(df), _ptr(out), self._stream()))
         return out
 
+    def image_db8(self, img: torch.Tensor, dyn_range_db: float = 70.0,
+                  out: torch.Tensor | None = None) -> torch.Tensor:
+        """8-bit dB display image (Ny, Nx) uint8 on the device, top row = far
+        range, quantised as the reference's writeBMPFile does
+        (uncc_sar_globals.hpp:66-118)."""
+        Nx, Ny = img.shape
+        _need(img, torch.complex64, (Nx, Ny), "img")
+        if out is None:
+            out = torch.empty((Ny, Nx), dtype=torch.uint8, device=img.device)
+        _need(out, torch.uint8, (Ny, Nx), "out")
+        check(self.lib.sarf_image_db8(self._h, _ptr(img), Nx, Ny, float(dyn_range_db), _ptr(out),
+                                      self._stream()))
+        return out
+
     def autotune(self, prof: torch.Tensor, ant: torch.Tensor, r0: torch.Tensor,
                  f0: torch.Tensor, grid: ImageGrid, maxWr: float, *, reps: int = 3,
                  out: torch.Tensor | None = None,

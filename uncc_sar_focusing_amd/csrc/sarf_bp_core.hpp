@@ -200,6 +200,10 @@ struct BpTile {
             const float sy = A.y * qy;
             const float wmax = B.w;
             if (I.w) {
+                // Interior: the window has one bin of margin on each side of
+                // [pos_c - span, pos_c + span] and span bounds |dl| * invbin,
+                // so 1 <= p <= W - 2 up to FP32 rounding (< 0.01 bin) and the
+                // truncating conversion needs no clamp.
 #pragma unroll
                 for (int j = 0; j < PX; j++) {
                     const float s = fmaf(A.x, qx[j], sy);
@@ -208,7 +212,7 @@ struct BpTile {
                     const float dl = fmaf(tq, g, -s);  // dR - dR_c
                     const float p = fmaf(dl, invbin, B.x);
                     const float fr = __builtin_amdgcn_fractf(p);
-                    const int i = (int)__builtin_amdgcn_fmed3f(p, 0.f, wmax);
+                    const int i = (int)p;
                     const float4 w = L.win[I.y + i];
                     const float vr = fmaf(fr, w.z, w.x);
                     const float vi = fmaf(fr, w.w, w.y);

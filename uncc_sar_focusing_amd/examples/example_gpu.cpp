@@ -4,7 +4,9 @@
 //   1. sph_sar_data_callback_gpu(...) on raw arrays -> BMP file;
 //   2. SAR_Aperture + SAR_ImageFormationParameters with zero padding and an
 //      explicit pixel grid -> cuda_focus_SAR_image -> brightest pixel, which
-//      must land on the strongest synthetic target.
+//      must land on the strongest synthetic target;
+//   3. optionally (argv[2], argv[3]) that image as raw complex64 (x-major)
+//      and as a BMP from writeBMPFile.
 // Prints one line "peak x=<m> y=<m> |I|=<value>" and exits 0 on success.
 #include <charlotte_sar_api.hpp>
 
@@ -71,5 +73,13 @@ int main(int argc, char **argv) {
     const double px = p.x0_m - p.Wx_m / 2 + bx * p.Wx_m / (p.N_x_pix - 1);
     const double py = p.y0_m - p.Wy_m / 2 + by * p.Wy_m / (p.N_y_pix - 1);
     std::printf("peak x=%.3f y=%.3f |I|=%.4f\n", px, py, Complex<float>::abs(img[best]));
+    if (argc > 3) {
+        std::FILE *f = std::fopen(argv[2], "wb");
+        if (!f) return 3;
+        std::fwrite(&img[0], sizeof(float) * 2, img.size(), f);
+        std::fclose(f);
+        p.dyn_range_dB = 70;
+        if (writeBMPFile(p, img, argv[3]) != EXIT_SUCCESS) return 3;
+    }
     return (std::fabs(px) < 1e-3 && std::fabs(py) < 1e-3) ? 0 : 2;
 }
