@@ -40,8 +40,10 @@ class PhaseHistory {
     int numAzimuthSamples;
 };
 
-// MAT-file input (charlotte_sar_api.cpp:36-148) is not part of this library
-// (no matio); it is declared so callers link, and returns EXIT_FAILURE.
+// MAT-file input (charlotte_sar_api.cpp:36-148): reads a Level 5 MAT-file
+// (Sandia sph_MATData.* or GOTCHA data.* record) with libsarf's own reader
+// and fills `ph` with the central min(MAX_PULSES, Np) pulses of the channel
+// named by `polarity` ("HH"/"any", "HV", "VH", "VV").
 int readData(const std::string &inputfile, const int MAX_PULSES, const std::string &polarity,
              PhaseHistory<float> &ph, bool verbose);
 

@@ -53,7 +53,7 @@ def build_library(verbose: bool = True) -> str:
         glob.glob(os.path.join(CSRC, "*.cpp"))
     )
     tmp = LIB_PATH + ".tmp"
-    cmd = [HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}", "-o", tmp, *srcs]
+    cmd = [HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}", "-o", tmp, *srcs, "-lz"]
     _run(cmd, verbose)
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
