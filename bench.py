@@ -90,7 +90,7 @@ def cpu_baseline(cfg, geom, grid, ph_host, target_s=4.0):
         return time.perf_counter() - t0
 
     def sized(nt, target):
-        P0 = 4 if nt > 1 else 1
+        P0 = 32 if nt > 1 else 2
         t_cal = run(P0, nt)
         return int(min(geom.Np, max(P0, round(P0 * target / max(t_cal, 1e-3)))))
 
@@ -112,6 +112,23 @@ def cpu_baseline(cfg, geom, grid, ph_host, target_s=4.0):
         "single_thread_value": rate1 / 1e9,
         "single_thread_sample": f"first {P1} pulses, 1 thread, median of 3 ({dts1[1]:.2f} s)",
     }
+
+
+def measured_traffic(config: str, shape):
+    """HBM bytes per bp_mixed launch from the committed FETCH_SIZE/WRITE_SIZE
+    summary, if it was measured on this workload with this launch shape."""
+    path = os.path.join(REPO, "profiles", f"bp_mixed_{config}_hbm_counters.md")
+    if not shape or not os.path.exists(path):
+        return None, None
+    vals = {}
+    for ln in open(path):
+        if ":" in ln and ln.split(":")[0] in ("traffic_bytes_per_launch", "shape"):
+            k, v = ln.split(":", 1)
+            vals[k] = v.strip()
+    want = f"tile {shape.get('tile')} split {shape.get('pulse_split')}"
+    if vals.get("shape") != want or "traffic_bytes_per_launch" not in vals:
+        return None, f"profiles/{os.path.basename(path)} is for {vals.get('shape')}, not {want}"
+    return int(vals["traffic_bytes_per_launch"]), f"profiles/{os.path.basename(path)}"
 
 
 def probe_main(args) -> int:
@@ -362,13 +379,15 @@ def main() -> int:
 
     # roofline of the backprojection kernel
     achieved = FLOPS_PER_PP * pp / (t_bp * 1e-3) / 1e12
+    traffic, traffic_src = measured_traffic(cfg.name, tune)
     roof = {
         "bound": "valu",
         "achieved": round(achieved, 3),
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-        "traffic": None,
+        "traffic": traffic,
+        "traffic_source": traffic_src,
         "kernel": "bp_mixed",
         "trans_frac": round(TRANS_PER_PP * pp / (t_bp * 1e-3) / 1e12 / PEAK_TRANS_TOPS, 4),
     }

@@ -1,7 +1,7 @@
 """GPU PGA against the numpy restatement of matlab/PGA.m (oracle.pga_f64) on
 the same complex64 input.  Tolerances: accumulated phase within 2e-3 rad,
 per-iteration rms within 1e-3, corrected image within 1e-3 relative L2
-(FP32 direct transforms vs FP64 FFTs through several data-dependent
+(FP32 Bluestein FFTs vs FP64 FFTs through several data-dependent
 iterations)."""
 import numpy as np
 import pytest
@@ -27,7 +27,7 @@ def _degraded_image(nx, ny, seed=4, amp=3.0):
     return img, bad.astype(np.complex64)
 
 
-@pytest.mark.parametrize("nx,ny", [(64, 64), (75, 50), (128, 97)])
+@pytest.mark.parametrize("nx,ny", [(64, 64), (75, 50), (128, 97), (333, 40), (1024, 31)])
 def test_pga_matches_oracle(ctx, nx, ny):
     img, bad = _degraded_image(nx, ny)
     iters = 3
@@ -38,8 +38,9 @@ def test_pga_matches_oracle(ctx, nx, ny):
     assert np.max(np.abs(af_ph.cpu().numpy() - ref_ph)) < 2e-3
     assert np.max(np.abs(rms.cpu().numpy() - np.array(ref_rms))) < 1e-3
     assert oracle.rel_l2(out, ref_img) < 1e-3
-    # and it focuses: entropy back near the undegraded image
-    assert oracle.entropy_f64(out) < oracle.entropy_f64(bad) - 0.3
+    assert abs(oracle.entropy_f64(out) - oracle.entropy_f64(ref_img)) < 1e-3
+    if nx <= 128:  # square-ish scenes: PGA focuses them (the wide ones need more rows)
+        assert oracle.entropy_f64(out) < oracle.entropy_f64(bad) - 0.3
 
 
 def test_pga_zero_iterations_is_identity(ctx):

@@ -404,7 +404,9 @@ extern "C" int sarf_autotune(sarf_ctx *ctx, const sarf_bp_desc *d, const void *p
             float ms = 0.f;
             SARF_TRY(time_shape(tile, splits, &ms));
             n++;
-            if (ms < best_ms) {
+            // 1 % hysteresis: timing noise alone does not move the pick
+            // away from the default or an earlier candidate
+            if (ms < 0.99f * best_ms) {
                 best_ms = ms;
                 best_tile = tile;
                 best_split = splits;

@@ -15,11 +15,19 @@
 //   5. phi = unwrap([0 cumsum(phi_dot)]), linear trend removed by least
 //      squares over x = 0..Nx-1, rms recorded, af_ph += phi (PGA.m:204-217);
 //   6. A <- ifft(fft(A) .* e^{-j phi}) along x (PGA.m:229-232).
-// GPU formulation: every pass reads rows in y-contiguous order; the shift,
-// mask and ifftshift of step 3 fold into an index remap plus one phase factor
-// per (k, y); step 6 is one circular convolution A <- A (*) m with
-// m = ifft(e^{-j phi}).  Transforms are direct sums with exact integer
-// twiddle indices (any Nx), FP32 products, FP64 reductions.
+//
+// GPU formulation.  The image is transposed once into rows along x
+// (At[y*Nx + x]) for all iterations and transposed back at the end.  Every
+// transform along x is a length-N DFT (any N <= 4096) done by Bluestein's
+// chirp-z identity inside one workgroup per row: nk = (n^2 + k^2 - (k-n)^2)/2
+// turns the DFT into a circular convolution with the chirp
+// w_m = exp(-j pi m^2 / N), evaluated with two power-of-two FFTs of length
+// M >= 2N - 1 on the row held in LDS (radix-2, in place); FFT(chirp) is
+// computed once per call.  The shift, mask and ifftshift of step 3 fold into
+// the load (index remap) and one phase factor per (k, y); step 6 is a
+// forward transform, the e^{-j phi} multiply and an inverse transform in the
+// same workgroup, in place.  Reductions over rows (column energy, phase
+// differences) are two-pass FP64 sums.
 #include <algorithm>
 #include <cmath>
 
@@ -29,69 +37,181 @@ namespace sarf {
 namespace {
 
 constexpr int kPgaMaxN = 4096;
+constexpr int kThreads = 256;
+constexpr int kRowChunks = 16;  // y chunks of the two-pass column reductions
 
-constexpr int kArgSeg = 16;  // x segments per row in the first argmax pass
+// In-place radix-2 decimation-in-time FFT of buf[0..M), M = 2^L, by one
+// workgroup of kThreads: forward uses e^{-j 2 pi t/M}, inverse e^{+j 2 pi t/M}
+// (unnormalised).  tw[t] = e^{+j 2 pi t / M}, t < M.
+__device__ void lds_fft(float2 *buf, int L, const float2 *__restrict__ tw, bool inverse) {
+    const int M = 1 << L;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < M; i += kThreads) {
+        const int r = (int)(__brev((unsigned)i) >> (32 - L));
+        if (r > i) {
+            const float2 t = buf[i];
+            buf[i] = buf[r];
+            buf[r] = t;
+        }
+    }
+    __syncthreads();
+    for (int s = 1; s <= L; s++) {
+        const int h = 1 << (s - 1);
+        const int tstride = M >> s;
+        for (int b = tid; b < M / 2; b += kThreads) {
+            const int j = b & (h - 1);
+            const int i0 = ((b >> (s - 1)) << s) + j;
+            const int i1 = i0 + h;
+            float2 w = tw[j * tstride];
+            if (!inverse) w.y = -w.y;
+            const float2 u = buf[i0];
+            const float2 v = cmul(buf[i1], w);
+            buf[i0] = make_float2(u.x + v.x, u.y + v.y);
+            buf[i1] = make_float2(u.x - v.x, u.y - v.y);
+        }
+        __syncthreads();
+    }
+}
 
-// Pass 1: per (segment, y) the first maximum of |A(y, x)|^2 over the
-// segment's x range; grid (ceil(Ny/256), kArgSeg).
-__global__ __launch_bounds__(256) void pga_row_argmax_seg(const float2 *__restrict__ A, int Nx,
-                                                          int Ny, float *__restrict__ segmax,
-                                                          int *__restrict__ segidx) {
-    const int y = blockIdx.x * blockDim.x + threadIdx.x;
-    const int seg = blockIdx.y;
-    if (y >= Ny) return;
-    const int per = (Nx + kArgSeg - 1) / kArgSeg;
-    const int xb = seg * per, xe = min(Nx, xb + per);
+// Bluestein chirp w_m = e^{-j pi m^2 / N} from ch2n[t] = e^{+j 2 pi t/(2N)}.
+__device__ __forceinline__ float2 chirp(int m, int N, const float2 *__restrict__ ch2n) {
+    const long t = ((long)m * m) % (2L * N);
+    const float2 c = ch2n[t];
+    return make_float2(c.x, -c.y);
+}
+
+// buf[0..N) holds a_n = x_n w_n, buf[N..M) zeros; on return buf[k] (k < N)
+// holds X_k / w_k * M, i.e. X_k = w_k buf[k] / M is the DFT of x.
+__device__ __forceinline__ void bluestein_core(float2 *buf, int L, const float2 *__restrict__ twM,
+                                               const float2 *__restrict__ H) {
+    const int M = 1 << L;
+    lds_fft(buf, L, twM, false);
+    for (int i = threadIdx.x; i < M; i += kThreads) buf[i] = cmul(buf[i], H[i]);
+    __syncthreads();
+    lds_fft(buf, L, twM, true);
+}
+
+// H = FFT_M(h), h_m = conj(w_m) for m < N, h_{M-m} = conj(w_m) for 0 < m < N.
+__global__ __launch_bounds__(kThreads) void pga_chirp_spectrum(int N, int L,
+                                                               const float2 *__restrict__ ch2n,
+                                                               const float2 *__restrict__ twM,
+                                                               float2 *__restrict__ H) {
+    extern __shared__ float2 buf[];
+    const int M = 1 << L;
+    for (int i = threadIdx.x; i < M; i += kThreads) {
+        float2 v = make_float2(0.f, 0.f);
+        if (i < N) v = chirp(i, N, ch2n);
+        else if (i > M - N) v = chirp(M - i, N, ch2n);
+        buf[i] = make_float2(v.x, -v.y);
+    }
+    __syncthreads();
+    lds_fft(buf, L, twM, false);
+    for (int i = threadIdx.x; i < M; i += kThreads) H[i] = buf[i];
+}
+
+// dst[c*R + r] = src[r*C + c] for an R x C complex matrix (64 x 64 tiles).
+__global__ __launch_bounds__(kThreads) void transpose_c64(const float2 *__restrict__ src, int R,
+                                                          int C, float2 *__restrict__ dst) {
+    __shared__ float2 tile[64][65];
+    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    for (int i = ly; i < 64; i += kThreads / 64) {
+        const int r = r0 + i, c = c0 + lx;
+        if (r < R && c < C) tile[i][lx] = src[(size_t)r * C + c];
+    }
+    __syncthreads();
+    for (int i = ly; i < 64; i += kThreads / 64) {
+        const int c = c0 + i, r = r0 + lx;
+        if (r < R && c < C) dst[(size_t)c * R + r] = tile[lx][i];
+    }
+}
+
+// Per row y (one workgroup): first maximum of |At[y][x]|^2 over x ->
+// circular shift s_y = ceil(N/2) - idx (mod N).
+__global__ __launch_bounds__(kThreads) void pga_row_argmax(const float2 *__restrict__ At, int N,
+                                                           int *__restrict__ shift) {
+    __shared__ float bm[kThreads];
+    __shared__ int bi[kThreads];
+    const int y = blockIdx.x, tid = threadIdx.x;
+    const float2 *row = At + (size_t)y * N;
     float best = -1.f;
-    int bi = xb;
-    for (int x = xb; x < xe; x++) {
-        const float2 v = A[(size_t)x * Ny + y];
+    int idx = 0;
+    for (int x = tid; x < N; x += kThreads) {
+        const float2 v = row[x];
         const float m = v.x * v.x + v.y * v.y;
         if (m > best) {
             best = m;
-            bi = x;
+            idx = x;
         }
     }
-    segmax[(size_t)seg * Ny + y] = best;
-    segidx[(size_t)seg * Ny + y] = bi;
-}
-
-// Pass 2: first maximum over the segments (in x order) -> circular shift
-// s_y = ceil(Nx/2) - idx_y (mod Nx).
-__global__ __launch_bounds__(256) void pga_row_shift(const float *__restrict__ segmax,
-                                                     const int *__restrict__ segidx, int Nx,
-                                                     int Ny, int *__restrict__ shift) {
-    const int y = blockIdx.x * blockDim.x + threadIdx.x;
-    if (y >= Ny) return;
-    float best = -1.f;
-    int bi = 0;
-    for (int s = 0; s < kArgSeg; s++) {
-        const float m = segmax[(size_t)s * Ny + y];
-        if (m > best) {
-            best = m;
-            bi = segidx[(size_t)s * Ny + y];
-        }
-    }
-    const int center = (Nx + 1) / 2;  // 0-based target column = 1-based ceil(Nx/2)
-    shift[y] = ((center - bi) % Nx + Nx) % Nx;
-}
-
-__global__ __launch_bounds__(256) void pga_col_energy(const float2 *__restrict__ A, int Nx, int Ny,
-                                                      double *__restrict__ w) {
-    __shared__ double red[256];
-    const int x = blockIdx.x;
-    double s = 0.0;
-    for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
-        const float2 v = A[(size_t)x * Ny + y];
-        s += (double)v.x * v.x + (double)v.y * v.y;
-    }
-    red[threadIdx.x] = s;
+    bm[tid] = best;
+    bi[tid] = idx;
     __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    for (int o = kThreads / 2; o > 0; o >>= 1) {
+        if (tid < o) {
+            const float m2 = bm[tid + o];
+            const int i2 = bi[tid + o];
+            if (m2 > bm[tid] || (m2 == bm[tid] && i2 < bi[tid])) {
+                bm[tid] = m2;
+                bi[tid] = i2;
+            }
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) w[x] = red[0];
+    if (tid == 0) {
+        const int center = (N + 1) / 2;  // 0-based target column = 1-based ceil(N/2)
+        shift[y] = ((center - bi[0]) % N + N) % N;
+    }
+}
+
+// Two-pass column sums over rows.  Pass 1, grid (ceil(N/64), kRowChunks):
+// part[chunk*N + x] = sum over the chunk's rows of |At[y][x]|^2 (mode 0), or
+// for G (mode 1) num/den parts of phi_dot: Im(conj G_k G_{k+1}), |G_k|^2.
+__global__ __launch_bounds__(kThreads) void pga_col_partial(const float2 *__restrict__ At, int N,
+                                                            int Ny, int mode,
+                                                            double *__restrict__ pa,
+                                                            double *__restrict__ pb) {
+    __shared__ double ra[4][64], rb[4][64];
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    const int x = blockIdx.x * 64 + lx;
+    const int per = (Ny + kRowChunks - 1) / kRowChunks;
+    const int yb = blockIdx.y * per, ye = min(Ny, yb + per);
+    double a = 0.0, b = 0.0;
+    if (x < N) {
+        for (int y = yb + ly; y < ye; y += 4) {
+            const float2 g = At[(size_t)y * N + x];
+            if (mode == 0) {
+                a += (double)g.x * g.x + (double)g.y * g.y;
+            } else if (x + 1 < N) {
+                const float2 g1 = At[(size_t)y * N + x + 1];
+                a += (double)g.x * g1.y - (double)g.y * g1.x;
+                b += (double)g.x * g.x + (double)g.y * g.y;
+            }
+        }
+    }
+    ra[ly][lx] = a;
+    rb[ly][lx] = b;
+    __syncthreads();
+    if (ly == 0 && x < N) {
+        pa[(size_t)blockIdx.y * N + x] = (ra[0][lx] + ra[1][lx]) + (ra[2][lx] + ra[3][lx]);
+        if (mode == 1) pb[(size_t)blockIdx.y * N + x] = (rb[0][lx] + rb[1][lx]) + (rb[2][lx] + rb[3][lx]);
+    }
+}
+
+// Pass 2: out[x] = sum of the chunk parts (mode 0: energy), or
+// phi_dot[k] = num / den for k < N-1 (mode 1).
+__global__ __launch_bounds__(kThreads) void pga_col_final(const double *__restrict__ pa,
+                                                          const double *__restrict__ pb, int N,
+                                                          int mode, double *__restrict__ out) {
+    const int x = blockIdx.x * kThreads + threadIdx.x;
+    if (x >= N) return;
+    double a = 0.0, b = 0.0;
+    for (int c = 0; c < kRowChunks; c++) {
+        a += pa[(size_t)c * N + x];
+        if (mode == 1) b += pb[(size_t)c * N + x];
+    }
+    if (mode == 0) out[x] = a;
+    else if (x < N - 1) out[x] = b > 0.0 ? a / b : 0.0;
 }
 
 // One workgroup: window limits from the column energy (mask[j] = 1 on
@@ -137,101 +257,82 @@ __global__ __launch_bounds__(1024) void pga_window(const double *__restrict__ w,
     for (int j = tid; j < Nx; j += blockDim.x) mask[j] = (j >= left && j <= right) ? 1.f : 0.f;
 }
 
-// G[k][y] = e^{-j 2 pi k (s_y - fl)/N} sum_x mask[(x + s_y) mod N] A[x][y] e^{-j 2 pi k x / N}
-// (fl = floor(N/2)).  A workgroup covers 64 y (one per lane) and KB
-// consecutive k; its four waves each sum a quarter of x and the quarters
-// are added through LDS, so a wave streams at most ceil(N/4) columns.
-template <int KB>
-__global__ __launch_bounds__(256) void pga_dft(const float2 *__restrict__ A, int Nx, int Ny,
-                                               const int *__restrict__ shift,
-                                               const float *__restrict__ mask,
-                                               const float2 *__restrict__ tw /* e^{+j 2 pi i/N} */,
-                                               float2 *__restrict__ G) {
-    __shared__ float2 stw[kPgaMaxN];
-    __shared__ float smask[kPgaMaxN];
-    __shared__ float2 red[3][KB][64];
-    for (int i = threadIdx.x; i < Nx; i += blockDim.x) {
-        stw[i] = tw[i];
-        smask[i] = mask[i];
-    }
-    __syncthreads();
-    const int ly = threadIdx.x & 63;
-    const int part = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int y = blockIdx.x * 64 + ly;
-    const int k0 = blockIdx.y * KB;
-    const bool live = y < Ny;
-    const int s = live ? shift[y] : 0;
-    const int per = (Nx + 3) / 4;
-    const int xb = min(Nx, part * per), xe = min(Nx, xb + per);
-    float ar[KB], ai[KB];
-    int idx[KB], step[KB];
-#pragma unroll
-    for (int q = 0; q < KB; q++) {
-        ar[q] = ai[q] = 0.f;
-        step[q] = (k0 + q) % Nx;  // k mod N keeps every index below N
-        idx[q] = (int)(((long)step[q] * xb) % Nx);
-    }
-    if (live) {
-        int mi = (xb + s) % Nx;  // (x + s) mod N
-        for (int x = xb; x < xe; x++) {
-            const float m = smask[mi];
-            if (++mi == Nx) mi = 0;
-            float2 v = A[(size_t)x * Ny + y];
-            v.x *= m;
-            v.y *= m;
-#pragma unroll
-            for (int q = 0; q < KB; q++) {
-                const float2 w = stw[idx[q]];  // e^{+j 2 pi k x/N}; use its conjugate
-                ar[q] = fmaf(v.x, w.x, fmaf(v.y, w.y, ar[q]));
-                ai[q] = fmaf(v.y, w.x, fmaf(-v.x, w.y, ai[q]));
-                idx[q] += step[q];
-                if (idx[q] >= Nx) idx[q] -= Nx;
-            }
+// Step 3 for row y (one workgroup, dynamic LDS of M float2):
+// G[y][k] = e^{-j 2 pi k (s - fl)/N} DFT_x( mask[(x + s) mod N] At[y][x] )_k,
+// fl = floor(N/2): the shift by s, the mask and ifftshift as one remap.
+__global__ __launch_bounds__(kThreads) void pga_row_dft(const float2 *__restrict__ At, int N, int L,
+                                                        const int *__restrict__ shift,
+                                                        const float *__restrict__ mask,
+                                                        const float2 *__restrict__ ch2n,
+                                                        const float2 *__restrict__ twM,
+                                                        const float2 *__restrict__ twN,
+                                                        const float2 *__restrict__ H,
+                                                        float2 *__restrict__ G) {
+    extern __shared__ float2 buf[];
+    const int y = blockIdx.x, M = 1 << L;
+    const int s = shift[y];
+    const float2 *row = At + (size_t)y * N;
+    for (int x = threadIdx.x; x < M; x += kThreads) {
+        float2 v = make_float2(0.f, 0.f);
+        if (x < N) {
+            int mi = x + s;
+            if (mi >= N) mi -= N;
+            const float m = mask[mi];
+            const float2 a = row[x];
+            v = cmul(make_float2(m * a.x, m * a.y), chirp(x, N, ch2n));
         }
-    }
-    if (part > 0) {
-#pragma unroll
-        for (int q = 0; q < KB; q++) red[part - 1][q][ly] = make_float2(ar[q], ai[q]);
+        buf[x] = v;
     }
     __syncthreads();
-    if (part != 0 || !live) return;
-    const int fl = Nx / 2;
-#pragma unroll
-    for (int q = 0; q < KB; q++) {
-        const int k = k0 + q;
-        if (k >= Nx) break;
-        const float2 p1 = red[0][q][ly], p2 = red[1][q][ly], p3 = red[2][q][ly];
-        const float sr = (ar[q] + p1.x) + (p2.x + p3.x);
-        const float si = (ai[q] + p1.y) + (p2.y + p3.y);
-        // e^{-j 2 pi k (s - fl) / N}
-        const long e = ((long)k * (((s - fl) % Nx + Nx) % Nx)) % Nx;
-        const float2 w = stw[e];
-        G[(size_t)k * Ny + y] = make_float2(sr * w.x + si * w.y, si * w.x - sr * w.y);
+    bluestein_core(buf, L, twM, H);
+    const float invM = 1.0f / (float)M;
+    const int fl = N / 2;
+    const long sh = ((long)(s - fl) % N + N) % N;
+    for (int k = threadIdx.x; k < N; k += kThreads) {
+        float2 X = cmul(buf[k], chirp(k, N, ch2n));
+        X.x *= invM;
+        X.y *= invM;
+        const float2 e = twN[((long)k * sh) % N];  // e^{+j 2 pi k sh / N}; use its conjugate
+        G[(size_t)y * N + k] = cmul(X, make_float2(e.x, -e.y));
     }
 }
 
-// num[k] = sum_y Im(conj(G_k) G_{k+1}), den[k] = sum_y |G_k|^2, k < N-1.
-__global__ __launch_bounds__(256) void pga_phidot(const float2 *__restrict__ G, int Nx, int Ny,
-                                                  double *__restrict__ phidot) {
-    __shared__ double rn[256], rd[256];
-    const int k = blockIdx.x;
-    double num = 0.0, den = 0.0;
-    for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
-        const float2 a = G[(size_t)k * Ny + y], b = G[(size_t)(k + 1) * Ny + y];
-        num += (double)a.x * b.y - (double)a.y * b.x;
-        den += (double)a.x * a.x + (double)a.y * a.y;
-    }
-    rn[threadIdx.x] = num;
-    rd[threadIdx.x] = den;
+// Step 6 for row y, in place: At[y] <- IDFT( DFT(At[y]) .* e^{-j phi} ),
+// the inverse as conj(DFT(conj(.)))/N through the same Bluestein core.
+__global__ __launch_bounds__(kThreads) void pga_row_apply(float2 *__restrict__ At, int N, int L,
+                                                          const double2 *__restrict__ ephi,
+                                                          const float2 *__restrict__ ch2n,
+                                                          const float2 *__restrict__ twM,
+                                                          const float2 *__restrict__ H) {
+    extern __shared__ float2 buf[];
+    const int y = blockIdx.x, M = 1 << L;
+    float2 *row = At + (size_t)y * N;
+    for (int x = threadIdx.x; x < M; x += kThreads)
+        buf[x] = x < N ? cmul(row[x], chirp(x, N, ch2n)) : make_float2(0.f, 0.f);
     __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            rn[threadIdx.x] += rn[threadIdx.x + o];
-            rd[threadIdx.x] += rd[threadIdx.x + o];
+    bluestein_core(buf, L, twM, H);
+    const float invM = 1.0f / (float)M;
+    // Z_k = X_k e^{-j phi_k}; next input conj(Z_k) w_k, zero-padded (each
+    // lane reads and rewrites only its own elements)
+    for (int k = threadIdx.x; k < M; k += kThreads) {
+        float2 v = make_float2(0.f, 0.f);
+        if (k < N) {
+            const float2 w = chirp(k, N, ch2n);
+            const float2 X = cmul(buf[k], w);
+            const double2 e = ephi[k];
+            const float2 Z = cmul(make_float2(X.x * invM, X.y * invM),
+                                  make_float2((float)e.x, (float)e.y));
+            v = cmul(make_float2(Z.x, -Z.y), w);
         }
-        __syncthreads();
+        buf[k] = v;
     }
-    if (threadIdx.x == 0) phidot[k] = rd[0] > 0.0 ? rn[0] / rd[0] : 0.0;
+    __syncthreads();
+    bluestein_core(buf, L, twM, H);
+    const float scale = invM / (float)N;
+    for (int n = threadIdx.x; n < N; n += kThreads) {
+        const float2 u = cmul(buf[n], chirp(n, N, ch2n));
+        row[n] = make_float2(u.x * scale, -u.y * scale);
+    }
 }
 
 constexpr int kIntThreads = 1024;  // pga_integrate block; kPgaMaxN / 1024 = 4 per thread
@@ -260,7 +361,7 @@ __device__ inline double block_sum_1024(double v, double *red) {
 }
 
 // One workgroup of 1024: phi = unwrap([0 cumsum(phidot)]), detrend, rms,
-// af_ph += phi; writes e^{-j phi} (detrended) for the correction kernel.
+// af_ph += phi; writes e^{-j phi} (detrended) for the correction.
 // unwrap of a cumulative sum is the cumulative sum of the unwrapped steps,
 // so phi is one block-wide prefix scan of unwrap_step(phidot).
 __global__ __launch_bounds__(kIntThreads) void pga_integrate(const double *__restrict__ phidot,
@@ -324,90 +425,6 @@ __global__ __launch_bounds__(kIntThreads) void pga_integrate(const double *__res
     if (tid == 0) *rms_out = sqrt(r / n);
 }
 
-// m[x] = (1/N) sum_k e^{-j phi_k} e^{+j 2 pi k x / N}  (ifft of e^{-j phi});
-// one workgroup per x, k strided over its lanes.
-__global__ __launch_bounds__(256) void pga_kernel(const double2 *__restrict__ ephi, int Nx,
-                                                  const double2 *__restrict__ tw,
-                                                  float2 *__restrict__ m) {
-    __shared__ double rr[256], ri[256];
-    const int x = blockIdx.x;
-    const int tid = threadIdx.x;
-    double ar = 0.0, ai = 0.0;
-    long idx = ((long)tid * x) % Nx;
-    const long inc = ((long)blockDim.x * x) % Nx;
-    for (int k = tid; k < Nx; k += blockDim.x) {
-        const double2 e = ephi[k];
-        const double2 w = tw[idx];
-        ar += e.x * w.x - e.y * w.y;
-        ai += e.x * w.y + e.y * w.x;
-        idx += inc;
-        if (idx >= Nx) idx -= Nx;
-    }
-    rr[tid] = ar;
-    ri[tid] = ai;
-    __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-        if (tid < o) {
-            rr[tid] += rr[tid + o];
-            ri[tid] += ri[tid + o];
-        }
-        __syncthreads();
-    }
-    if (tid == 0) m[x] = make_float2((float)(rr[0] / Nx), (float)(ri[0] / Nx));
-}
-
-// out[x][y] = sum_x' m[(x - x') mod N] A[x'][y].  A workgroup covers 64 y
-// and XB consecutive output columns; its four waves each sum a quarter of
-// x' and the quarters are added through LDS.
-template <int XB>
-__global__ __launch_bounds__(256) void pga_apply(const float2 *__restrict__ A, int Nx, int Ny,
-                                                 const float2 *__restrict__ m,
-                                                 float2 *__restrict__ out) {
-    __shared__ float2 sm[kPgaMaxN];
-    __shared__ float2 red[3][XB][64];
-    for (int i = threadIdx.x; i < Nx; i += blockDim.x) sm[i] = m[i];
-    __syncthreads();
-    const int ly = threadIdx.x & 63;
-    const int part = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int y = blockIdx.x * 64 + ly;
-    const int x0 = blockIdx.y * XB;
-    const bool live = y < Ny;
-    const int per = (Nx + 3) / 4;
-    const int xb = min(Nx, part * per), xe = min(Nx, xb + per);
-    float ar[XB], ai[XB];
-    int idx[XB];
-#pragma unroll
-    for (int q = 0; q < XB; q++) {
-        ar[q] = ai[q] = 0.f;
-        idx[q] = ((x0 + q - xb) % Nx + Nx) % Nx;  // (x - x') mod N at x' = xb
-    }
-    if (live) {
-        for (int xp = xb; xp < xe; xp++) {
-            const float2 v = A[(size_t)xp * Ny + y];
-#pragma unroll
-            for (int q = 0; q < XB; q++) {
-                const float2 w = sm[idx[q]];
-                ar[q] = fmaf(v.x, w.x, fmaf(-v.y, w.y, ar[q]));
-                ai[q] = fmaf(v.x, w.y, fmaf(v.y, w.x, ai[q]));
-                if (--idx[q] < 0) idx[q] += Nx;
-            }
-        }
-    }
-    if (part > 0) {
-#pragma unroll
-        for (int q = 0; q < XB; q++) red[part - 1][q][ly] = make_float2(ar[q], ai[q]);
-    }
-    __syncthreads();
-    if (part != 0 || !live) return;
-#pragma unroll
-    for (int q = 0; q < XB; q++) {
-        if (x0 + q >= Nx) break;
-        const float2 p1 = red[0][q][ly], p2 = red[1][q][ly], p3 = red[2][q][ly];
-        out[(size_t)(x0 + q) * Ny + y] =
-            make_float2((ar[q] + p1.x) + (p2.x + p3.x), (ai[q] + p1.y) + (p2.y + p3.y));
-    }
-}
-
 __global__ void zero_f64(double *p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
@@ -429,64 +446,69 @@ extern "C" int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *d, void *img, double
     if (Nx > kPgaMaxN)
         return fail(SARF_ERR_UNSUPPORTED, "sarf_pga: Nx=%d > %d", Nx, kPgaMaxN);
     hipStream_t s = (hipStream_t)stream;
+    const int N = Nx;
+    int L = 1;
+    while ((1 << L) < 2 * N - 1) L++;
+    const int M = 1 << L;  // <= 8192
     const size_t npix = (size_t)Nx * Ny;
-    void *bufG, *bufT, *bufI;
+    const size_t lds = sizeof(float2) * (size_t)M;
+    void *bufA, *bufG, *bufS;
+    SARF_TRY(scratch_get(ctx, "pga.At", sizeof(float2) * npix, &bufA));
     SARF_TRY(scratch_get(ctx, "pga.G", sizeof(float2) * npix, &bufG));
-    SARF_TRY(scratch_get(ctx, "pga.tmp", sizeof(float2) * npix, &bufT));
     auto align16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
-    const size_t o_segmax = align16(sizeof(int) * Ny);
-    const size_t o_segidx = o_segmax + align16(sizeof(float) * kArgSeg * Ny);
-    const size_t o_w = o_segidx + align16(sizeof(int) * kArgSeg * Ny);
-    const size_t o_phidot = o_w + sizeof(double) * Nx;
-    const size_t o_ephi = o_phidot + sizeof(double) * Nx;
-    const size_t o_mask = o_ephi + sizeof(double2) * Nx;
-    const size_t o_m = o_mask + align16(sizeof(float) * Nx);
-    const size_t small = o_m + sizeof(float2) * Nx;
-    SARF_TRY(scratch_get(ctx, "pga.small", small, &bufI));
-    char *base = (char *)bufI;
+    const size_t o_parta = align16(sizeof(int) * Ny);
+    const size_t o_partb = o_parta + sizeof(double) * kRowChunks * N;
+    const size_t o_w = o_partb + sizeof(double) * kRowChunks * N;
+    const size_t o_phidot = o_w + sizeof(double) * N;
+    const size_t o_ephi = o_phidot + sizeof(double) * N;
+    const size_t o_mask = o_ephi + sizeof(double2) * N;
+    const size_t o_H = o_mask + align16(sizeof(float) * N);
+    const size_t small = o_H + sizeof(float2) * M;
+    SARF_TRY(scratch_get(ctx, "pga.small", small, &bufS));
+    char *base = (char *)bufS;
     int *shift = (int *)base;
-    float *segmax = (float *)(base + o_segmax);
-    int *segidx = (int *)(base + o_segidx);
+    double *pa = (double *)(base + o_parta);
+    double *pb = (double *)(base + o_partb);
     double *w = (double *)(base + o_w);
     double *phidot = (double *)(base + o_phidot);
     double2 *ephi = (double2 *)(base + o_ephi);
     float *mask = (float *)(base + o_mask);
-    float2 *m = (float2 *)(base + o_m);
-    const float2 *tw32 = nullptr;
-    const double2 *tw64 = nullptr;
-    SARF_TRY(twiddles_any_f32(ctx, Nx, &tw32));
-    SARF_TRY(twiddles_any_f64(ctx, Nx, &tw64));
-    float2 *G = (float2 *)bufG;
-    // ping-pong between the image and the scratch copy; the result is
-    // copied back to the image once at the end when it lands in the copy
-    float2 *src = (float2 *)img, *dst = (float2 *)bufT;
+    float2 *H = (float2 *)(base + o_H);
+    const float2 *twM = nullptr, *twN = nullptr, *ch2n = nullptr;
+    SARF_TRY(twiddles_any_f32(ctx, M, &twM));
+    SARF_TRY(twiddles_any_f32(ctx, N, &twN));
+    SARF_TRY(twiddles_any_f32(ctx, 2L * N, &ch2n));
+    float2 *A = (float2 *)img, *At = (float2 *)bufA, *G = (float2 *)bufG;
+
     hipLaunchKernelGGL(zero_f64, dim3((Nx + 255) / 256), dim3(256), 0, s, af_ph, Nx);
-    const bool wide = npix >= ((size_t)1 << 18);
-    const int KB = wide ? 16 : 8;
-    const dim3 gy((Ny + 255) / 256);
-    const dim3 gxy((Ny + 63) / 64, (Nx + KB - 1) / KB);
-    for (int it = 0; it < d->iters; it++) {
-        hipLaunchKernelGGL(pga_row_argmax_seg, dim3(gy.x, kArgSeg), dim3(256), 0, s, src, Nx, Ny,
-                           segmax, segidx);
-        hipLaunchKernelGGL(pga_row_shift, gy, dim3(256), 0, s, segmax, segidx, Nx, Ny, shift);
-        hipLaunchKernelGGL(pga_col_energy, dim3(Nx), dim3(256), 0, s, src, Nx, Ny, w);
-        hipLaunchKernelGGL(pga_window, dim3(1), dim3(1024), 0, s, w, Nx, d->db_down, mask);
-        if (wide)
-            hipLaunchKernelGGL((pga_dft<16>), gxy, dim3(256), 0, s, src, Nx, Ny, shift, mask, tw32, G);
-        else
-            hipLaunchKernelGGL((pga_dft<8>), gxy, dim3(256), 0, s, src, Nx, Ny, shift, mask, tw32, G);
-        hipLaunchKernelGGL(pga_phidot, dim3(Nx - 1), dim3(256), 0, s, G, Nx, Ny, phidot);
-        hipLaunchKernelGGL(pga_integrate, dim3(1), dim3(kIntThreads), 0, s, phidot, Nx, ephi, af_ph,
-                           rms + it);
-        hipLaunchKernelGGL(pga_kernel, dim3(Nx), dim3(256), 0, s, ephi, Nx, tw64, m);
-        if (wide)
-            hipLaunchKernelGGL((pga_apply<16>), gxy, dim3(256), 0, s, src, Nx, Ny, m, dst);
-        else
-            hipLaunchKernelGGL((pga_apply<8>), gxy, dim3(256), 0, s, src, Nx, Ny, m, dst);
+    if (d->iters == 0) {
         SARF_HIP(hipGetLastError());
-        std::swap(src, dst);
+        return SARF_OK;
     }
-    if (src != (float2 *)img)
-        SARF_HIP(hipMemcpyAsync(img, src, sizeof(float2) * npix, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(pga_chirp_spectrum, dim3(1), dim3(kThreads), lds, s, N, L, ch2n, twM, H);
+    // img is Nx rows (x) of Ny values; At is Ny rows of Nx values
+    hipLaunchKernelGGL(transpose_c64, dim3((Ny + 63) / 64, (Nx + 63) / 64), dim3(kThreads), 0, s,
+                       A, Nx, Ny, At);
+    const dim3 gcol((N + 63) / 64, kRowChunks);
+    for (int it = 0; it < d->iters; it++) {
+        hipLaunchKernelGGL(pga_row_argmax, dim3(Ny), dim3(kThreads), 0, s, At, N, shift);
+        hipLaunchKernelGGL(pga_col_partial, gcol, dim3(kThreads), 0, s, At, N, Ny, 0, pa, pb);
+        hipLaunchKernelGGL(pga_col_final, dim3((N + kThreads - 1) / kThreads), dim3(kThreads), 0, s,
+                           pa, pb, N, 0, w);
+        hipLaunchKernelGGL(pga_window, dim3(1), dim3(1024), 0, s, w, N, d->db_down, mask);
+        hipLaunchKernelGGL(pga_row_dft, dim3(Ny), dim3(kThreads), lds, s, At, N, L, shift, mask,
+                           ch2n, twM, twN, H, G);
+        hipLaunchKernelGGL(pga_col_partial, gcol, dim3(kThreads), 0, s, G, N, Ny, 1, pa, pb);
+        hipLaunchKernelGGL(pga_col_final, dim3((N + kThreads - 1) / kThreads), dim3(kThreads), 0, s,
+                           pa, pb, N, 1, phidot);
+        hipLaunchKernelGGL(pga_integrate, dim3(1), dim3(kIntThreads), 0, s, phidot, N, ephi, af_ph,
+                           rms + it);
+        hipLaunchKernelGGL(pga_row_apply, dim3(Ny), dim3(kThreads), lds, s, At, N, L, ephi, ch2n,
+                           twM, H);
+        SARF_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(transpose_c64, dim3((Nx + 63) / 64, (Ny + 63) / 64), dim3(kThreads), 0, s,
+                       At, Ny, Nx, A);
+    SARF_HIP(hipGetLastError());
     return SARF_OK;
 }
