@@ -18,9 +18,10 @@
 // (d = dR - dR_c, q = p - p_c) to get the tile's share
 //   g_n = uh_c (A - B/R_c) - (Cx, Cy, 0)/R_c.
 // Lanes accumulate 4 pulses x 4 sums in registers; a reduce-scatter
-// butterfly over the wave leaves one sum per lane, then 4 waves add in LDS.
-// Tile shares land in partial[n][tile] (FP64) and are summed in tile order
-// by a second kernel (deterministic).  Pulse ranges are split over
+// butterfly over the wave leaves one sum per lane, and each wave writes its
+// own share (no workgroup barrier per pulse group).  Shares land in
+// partial[n][tile][wave] (FP64) and are summed in order by a second kernel
+// (deterministic).  Pulse ranges are split over
 // workgroups when the image has too few tiles to fill the GPU.
 #include <algorithm>
 #include <cmath>
@@ -47,11 +48,17 @@ __device__ __forceinline__ float wave_reduce_scatter(float (&v)[kNv], int lane) 
     for (int step = 0; step < kLogNv; step++) {
         const int o = 1 << step;
         const int half = (kNv / 2) >> step;
-        const bool up = (lane & o) != 0;
+        // all-ones in lanes that keep the upper half.  Kept opaque so the
+        // choice stays a bit blend of two registers: written as a select of
+        // v[i] / v[i + half] the compiler folds it into a lane-variable
+        // index into v, which costs a compare/select chain over all kNv.
+        unsigned m = 0u - (unsigned)((lane >> step) & 1);
+        __asm__ volatile("" : "+v"(m));
 #pragma unroll
         for (int i = 0; i < half; i++) {
-            const float keep = up ? v[i + half] : v[i];
-            const float send = up ? v[i] : v[i + half];
+            const unsigned lo = __float_as_uint(v[i]), hi = __float_as_uint(v[i + half]);
+            const float keep = __uint_as_float((hi & m) | (lo & ~m));
+            const float send = __uint_as_float((lo & m) | (hi & ~m));
             v[i] = keep + __shfl_xor(send, o, 64);
         }
     }
@@ -62,7 +69,9 @@ __device__ __forceinline__ float wave_reduce_scatter(float (&v)[kNv], int lane) 
 }
 
 // Value index held by lane l (< kNv) after wave_reduce_scatter: bit b of l
-// (step b kept the upper half) has weight kNv / 2^(b+1).
+// (step b kept the upper half) has weight kNv / 2^(b+1), i.e. the index is
+// l bit-reversed over log2(kNv) bits; the map is its own inverse, so value v
+// is held by lane scatter_index(v).
 __device__ __forceinline__ int scatter_index(int lane) {
     int idx = 0;
 #pragma unroll
@@ -71,6 +80,8 @@ __device__ __forceinline__ int scatter_index(int lane) {
     return idx;
 }
 
+constexpr int kWaves = kBpThreads / 64;
+
 template <int TY, int PX>
 __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
     BpArgs a, const float2 *__restrict__ prof, const double *__restrict__ ant,
@@ -78,7 +89,6 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
     const double *__restrict__ ent, double *__restrict__ partial) {
     __shared__ BpLds L;
     __shared__ float uz[kBpChunk];
-    __shared__ float red[kBpThreads / 64][kNv];
     // blockIdx.x = split * ntiles + tile; each split covers a pulse range
     const int ntiles = a.tiles_x * a.tiles_y;
     const int split = blockIdx.x / ntiles;
@@ -169,22 +179,17 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
                 }
             }
             const float r = wave_reduce_scatter(acc, lane);
-            __syncthreads();  // previous users of red are done
-            if (lane < kNv) red[wave][scatter_index(lane)] = r;
-            __syncthreads();
-            if (threadIdx.x < kSub && c0 + (int)threadIdx.x < nc) {
-                const int k = threadIdx.x, c = c0 + k;
-                float s4[4];
+            // lane k < kSub gathers pulse c0 + k's four sums of this wave
+            const int kk = lane & (kSub - 1);
+            float s4[4];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    float v = 0.f;
-                    for (int w = 0; w < kBpThreads / 64; w++) v += red[w][4 * k + q];
-                    s4[q] = v;
-                }
+            for (int q = 0; q < 4; q++) s4[q] = __shfl(r, scatter_index(4 * kk + q), 64);
+            if (lane < kSub && c0 + lane < nc) {
+                const int c = c0 + lane;
                 const float4 A = L.pA[c];
                 const double invR = 2.0 * (double)A.z;
                 const double base = (double)s4[0] - (double)s4[1] * invR;
-                double *o = partial + ((size_t)(n0 + c) * ntiles + t) * 3;
+                double *o = partial + (((size_t)(n0 + c) * ntiles + t) * kWaves + wave) * 3;
                 o[0] = (double)A.x * base - (double)s4[2] * invR;
                 o[1] = (double)A.y * base - (double)s4[3] * invR;
                 o[2] = (double)uz[c] * base;
@@ -194,15 +199,15 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
     }
 }
 
-// grad[n] = sum over tiles (in tile order) of partial[n][tile].
+// grad[n] = sum over the nshares (tile, wave) shares of partial[n][.].
 __global__ __launch_bounds__(64) void adjoint_reduce(const double *__restrict__ partial,
-                                                     int ntiles, int Np,
+                                                     int nshares, int Np,
                                                      double *__restrict__ grad) {
     const int n = blockIdx.x;
     const int lane = threadIdx.x;
     double g[3] = {0.0, 0.0, 0.0};
-    for (int t = lane; t < ntiles; t += 64) {
-        const double *p = partial + ((size_t)n * ntiles + t) * 3;
+    for (int t = lane; t < nshares; t += 64) {
+        const double *p = partial + ((size_t)n * nshares + t) * 3;
         g[0] += p[0];
         g[1] += p[1];
         g[2] += p[2];
@@ -242,15 +247,14 @@ extern "C" int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *d, const 
         return fail(SARF_ERR_INVALID, "sarf_bp_entropy_grad: null argument");
     if (d->precision != SARF_BP_MIXED)
         return fail(SARF_ERR_UNSUPPORTED, "sarf_bp_entropy_grad: mixed precision only");
-    // default: the forward kernel's choice, narrowed to 4 pixels per lane
-    // (tile 4 -> 5, 3 -> 2) to keep the adjoint's registers in check
+    // default: the forward kernel's choice (tiles 6 and 7, 16 pixels per
+    // lane, are not instantiated here and fall back to 4)
     int tile = d->tile;
     if (tile <= 0) {
         BpArgs tmp{};
         SARF_TRY(bp_fill_args(ctx, d, 0, &tmp));
         tile = bp_pick_tile(d, tmp.dx, tmp.dy);
-        if (tile == 4) tile = 5;
-        if (tile == 3) tile = 2;
+        if (tile == 6 || tile == 7) tile = 4;
     }
     BpArgs a{};
     SARF_TRY(bp_fill_args(ctx, d, tile, &a));
@@ -267,7 +271,8 @@ extern "C" int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *d, const 
     }
     a.splits = std::max(1, std::min(splits, d->Np));
     void *buf = nullptr;
-    SARF_TRY(scratch_get(ctx, "adjoint.partials", sizeof(double) * 3 * (size_t)ntiles * d->Np, &buf));
+    SARF_TRY(scratch_get(ctx, "adjoint.partials",
+                         sizeof(double) * 3 * (size_t)ntiles * kWaves * d->Np, &buf));
     double *partial = (double *)buf;
     hipStream_t s = (hipStream_t)stream;
     const float2 *p = (const float2 *)prof;
@@ -283,7 +288,8 @@ extern "C" int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *d, const 
             return fail(SARF_ERR_UNSUPPORTED, "sarf_bp_entropy_grad: tile %d not compiled", tid);
     }
     SARF_HIP(hipGetLastError());
-    hipLaunchKernelGGL(adjoint_reduce, dim3(d->Np), dim3(64), 0, s, partial, ntiles, d->Np, grad);
+    hipLaunchKernelGGL(adjoint_reduce, dim3(d->Np), dim3(64), 0, s, partial, ntiles * kWaves,
+                       d->Np, grad);
     SARF_HIP(hipGetLastError());
     return SARF_OK;
 }
