@@ -122,13 +122,14 @@ def measured_traffic(config: str, shape):
         return None, None
     vals = {}
     for ln in open(path):
-        if ":" in ln and ln.split(":")[0] in ("traffic_bytes_per_launch", "shape"):
-            k, v = ln.split(":", 1)
-            vals[k] = v.strip()
-    want = f"tile {shape.get('tile')} split {shape.get('pulse_split')}"
-    if vals.get("shape") != want or "traffic_bytes_per_launch" not in vals:
-        return None, f"profiles/{os.path.basename(path)} is for {vals.get('shape')}, not {want}"
-    return int(vals["traffic_bytes_per_launch"]), f"profiles/{os.path.basename(path)}"
+        if ln.startswith("traffic_bytes_per_launch[") and "]:" in ln:
+            k, v = ln.split("]:", 1)
+            vals[k[len("traffic_bytes_per_launch["):]] = int(v)
+    want = f"{shape.get('tile')},{shape.get('pulse_split')}"
+    src = f"profiles/{os.path.basename(path)}"
+    if want not in vals:
+        return None, f"{src} has no counters for tile,splits {want} (has {sorted(vals)})"
+    return vals[want], f"{src} (tile,splits {want})"
 
 
 def probe_main(args) -> int:
@@ -502,6 +503,16 @@ def autofocus_leg(ctx, torch):
         "E_nominal": entropy_at(np.zeros(3)),
         "E_true": entropy_at(sc.true_coef),
     }
+    # image precision of the backprojection engine on this configuration:
+    # the focused (true-trajectory) image, mixed vs all-FP64 GPU path
+    ant_true = t(af.apply_los_offsets(g.ant, sc.true_coef, af.poly_basis(g.Np),
+                                      af.los_directions(g.ant, grid.x0, grid.y0)))
+    img32 = ctx.backproject(prof, ant_true, sc.r0, sc.f0, grid, g.maxWr, los=los)
+    prof64 = ctx.range_compress(sc.ph.to(torch.complex128), sc.cfg.nfft)
+    img64 = ctx.backproject(prof64, ant_true, sc.r0, sc.f0, grid, g.maxWr, precision="f64")
+    out["rel_l2_vs_fp64"] = float(torch.linalg.vector_norm(img32.to(torch.complex128) - img64)
+                                  / torch.linalg.vector_norm(img64))
+    del prof64, img64
     log(f"bench: grid search {res.ncand} candidates in {res.gpu_ms:.1f} ms GPU "
         f"({out['grid_Gpixel_pulses_per_s']} Gpixel*pulses/s); best {out['grid_best_coef']} "
         f"true {out['true_coef']}; E grid {res.best_entropy:.4f} nominal {out['E_nominal']:.4f} "
@@ -600,6 +611,52 @@ def serving_leg(ctx, torch, n_scenes: int = 64, pga_iters: int = 5, n_streams: i
         b.synchronize()
         lat.append(a.elapsed_time(b))
     lat = np.array(lat)
+    # the same batch captured once into a HIP graph and replayed
+    order = sorted(range(len(imgs)), key=lambda i: -imgs[i].numel())
+
+    def batch_graph():
+        for s, p in zip(scenes, profs):
+            ctx.range_compress(s.ph, s.nfft, out=p)
+        ctx.backproject_scenes(descs)
+        ev = torch.cuda.Event()
+        ev.record()
+        for st in streams:
+            st.wait_event(ev)
+        for j, i in enumerate(order):
+            with torch.cuda.stream(streams[j % n_streams]):
+                pctx[j % n_streams].pga(imgs[i], pga_iters)
+        cur = torch.cuda.current_stream()
+        for st in streams:
+            e = torch.cuda.Event()
+            e.record(st)
+            cur.wait_event(e)
+
+    t_graph = None
+    try:
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            batch_graph()
+        graph.replay()
+        torch.cuda.synchronize()
+        e0.record()
+        graph.replay()
+        e1.record()
+        e1.synchronize()
+        t_graph = e0.elapsed_time(e1)
+    except RuntimeError as e:  # capture unsupported here: report, keep the eager numbers
+        log(f"bench: serving graph capture failed: {e}")
+    # precision of the batched backprojection (before PGA) against the
+    # all-FP64 path, every scene
+    for s, p in zip(scenes, profs):
+        ctx.range_compress(s.ph, s.nfft, out=p)
+    ctx.backproject_scenes(descs)
+    rels = []
+    for i, s in enumerate(scenes):
+        p64 = ctx.range_compress(s.ph.to(torch.complex128), s.nfft)
+        i64 = ctx.backproject(p64, s.ant, s.r0, s.f0, s.grid, s.geom.maxWr, precision="f64")
+        rels.append(float(torch.linalg.vector_norm(imgs[i].to(torch.complex128) - i64)
+                          / torch.linalg.vector_norm(i64)))
+        del p64, i64
     # focus check on the first scenes (outside the timed loop)
     ent_before, ent_after = [], []
     for i, s in enumerate(scenes[:8]):
@@ -620,6 +677,9 @@ def serving_leg(ctx, torch, n_scenes: int = 64, pga_iters: int = 5, n_streams: i
         "batch_Gpixel_pulses_per_s": round(total_pp / (t_batch * 1e-3) / 1e9, 1),
         "batch_rc_bp_Gpixel_pulses_per_s": round(total_pp / (t_rcbp * 1e-3) / 1e9, 1),
         "pga_streams": n_streams,
+        "batch_graph_replay_gpu_ms": None if t_graph is None else round(t_graph, 3),
+        "bp_rel_l2_vs_fp64_max": max(rels),
+        "bp_rel_l2_vs_fp64_median": float(np.median(rels)),
         "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
         "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
         "entropy_before_pga_first8": [round(v, 4) for v in ent_before],
@@ -627,8 +687,9 @@ def serving_leg(ctx, torch, n_scenes: int = 64, pga_iters: int = 5, n_streams: i
     }
     log(f"bench: serving {n_scenes} scenes in {t_batch:.1f} ms GPU "
         f"({out['batch_Gpixel_pulses_per_s']} Gpixel*pulses/s incl. PGA, RC+BP "
-        f"{out['batch_rc_bp_Gpixel_pulses_per_s']}); latency p50 {out['latency_ms_p50']} ms "
-        f"p99 {out['latency_ms_p99']} ms")
+        f"{out['batch_rc_bp_Gpixel_pulses_per_s']}); graph replay {out['batch_graph_replay_gpu_ms']} ms; "
+        f"latency p50 {out['latency_ms_p50']} ms p99 {out['latency_ms_p99']} ms; "
+        f"BP rel-L2 vs FP64 max {out['bp_rel_l2_vs_fp64_max']:.2e}")
     for c in pctx:
         c.close()
     return out

@@ -238,6 +238,11 @@ int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *desc,
 /* through an atomic counter).  Each scene is described as for              */
 /* sarf_backproject, with its own device buffers; images are overwritten.   */
 /* tile: 0 = chosen for the finest-spaced scene, else a tile id 1..5.       */
+/* The work list is uploaded (and the call waits for the copy) only when    */
+/* the descriptors or tile differ from the previous call on this context;   */
+/* a repeated call only enqueues kernels, so after one call outside a       */
+/* stream capture the same call can be captured into a HIP graph (a new     */
+/* list during capture returns SARF_ERR_INVALID).                           */
 /* The reference focuses one aperture per call (cuda_focus_SAR_image,       */
 /* cuda_sar_focusing.hpp:526-723).                                          */
 /* ------------------------------------------------------------------------ */

@@ -10,6 +10,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "sarf.h"
 
@@ -57,6 +58,10 @@ struct sarf_ctx {
     // backprojection launch shapes chosen by sarf_autotune, keyed by the
     // image geometry: (tile id, pulse splits)
     std::map<std::array<uint64_t, 8>, std::pair<int, int>> tuned;
+    // last scene list uploaded by sarf_bp_scenes (caller's descriptors and
+    // tile), so an identical call re-launches without a copy or a wait
+    std::vector<unsigned char> scenes_key;
+    void *scenes_buf = nullptr;
 };
 
 namespace sarf {

@@ -11,6 +11,7 @@
 // sarf_backproject and writes its tile of that scene's image.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "sarf_bp_core.hpp"
@@ -137,11 +138,34 @@ extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scen
     int2 *ditems = (int2 *)(((uintptr_t)((char *)buf + bytes_sc) + 15) & ~(uintptr_t)15);
     unsigned int *counter = (unsigned int *)(((uintptr_t)((char *)ditems + bytes_it) + 15) & ~(uintptr_t)15);
     hipStream_t s = (hipStream_t)stream;
-    // pageable source: the copy completes before returning, so the host
-    // vectors may go out of scope
-    SARF_HIP(hipMemcpyAsync(dsc, host.data(), bytes_sc, hipMemcpyHostToDevice, s));
-    SARF_HIP(hipMemcpyAsync(ditems, hitems.data(), bytes_it, hipMemcpyHostToDevice, s));
-    SARF_HIP(hipStreamSynchronize(s));
+    // The work list depends only on the caller's descriptors and the tile:
+    // when both match the last upload into this buffer, launch again
+    // without copying (and without waiting, so the call can be captured
+    // into a graph once the list has been uploaded outside the capture).
+    std::vector<unsigned char> key(sizeof(int) + sizeof(sarf_scene) * (size_t)nscenes);
+    std::memcpy(key.data(), &tid, sizeof(int));
+    std::memcpy(key.data() + sizeof(int), scenes, sizeof(sarf_scene) * (size_t)nscenes);
+    bool cached;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        cached = ctx->scenes_buf == buf && ctx->scenes_key == key;
+    }
+    if (!cached) {
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        SARF_HIP(hipStreamIsCapturing(s, &cap));
+        if (cap != hipStreamCaptureStatusNone)
+            return fail(SARF_ERR_INVALID,
+                        "sarf_bp_scenes: a new scene list cannot be uploaded during stream "
+                        "capture; make the same call once before capturing");
+        // pageable source: the copy completes before returning, so the host
+        // vectors may go out of scope
+        SARF_HIP(hipMemcpyAsync(dsc, host.data(), bytes_sc, hipMemcpyHostToDevice, s));
+        SARF_HIP(hipMemcpyAsync(ditems, hitems.data(), bytes_it, hipMemcpyHostToDevice, s));
+        SARF_HIP(hipStreamSynchronize(s));
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->scenes_buf = buf;
+        ctx->scenes_key = std::move(key);
+    }
     const int cus = ctx->cu_count > 0 ? ctx->cu_count : 256;
     const int nit = (int)hitems.size();
     const int grid = std::min(nit, 4 * cus);
