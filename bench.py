@@ -569,10 +569,15 @@ def serving_leg(ctx, torch, n_scenes: int = 64, pga_iters: int = 5, n_streams: i
     descs = [(profs[i], s.ant, s.r0, s.f0, s.grid, s.geom.maxWr, imgs[i])
              for i, s in enumerate(scenes)]
 
+    # batch tile for the one-launch backprojection of all scenes
+    for s, p in zip(scenes, profs):
+        ctx.range_compress(s.ph, s.nfft, out=p)
+    scene_tile, _ = ctx.autotune_scenes(descs)
+
     def batch():
         for s, p in zip(scenes, profs):
             ctx.range_compress(s.ph, s.nfft, out=p)
-        ctx.backproject_scenes(descs)
+        ctx.backproject_scenes(descs, tile=scene_tile)
         e_bp = torch.cuda.Event(enable_timing=True)
         e_bp.record()
         for st in streams:
@@ -617,7 +622,7 @@ def serving_leg(ctx, torch, n_scenes: int = 64, pga_iters: int = 5, n_streams: i
     def batch_graph():
         for s, p in zip(scenes, profs):
             ctx.range_compress(s.ph, s.nfft, out=p)
-        ctx.backproject_scenes(descs)
+        ctx.backproject_scenes(descs, tile=scene_tile)
         ev = torch.cuda.Event()
         ev.record()
         for st in streams:
@@ -649,7 +654,7 @@ def serving_leg(ctx, torch, n_scenes: int = 64, pga_iters: int = 5, n_streams: i
     # all-FP64 path, every scene
     for s, p in zip(scenes, profs):
         ctx.range_compress(s.ph, s.nfft, out=p)
-    ctx.backproject_scenes(descs)
+    ctx.backproject_scenes(descs, tile=scene_tile)
     rels = []
     for i, s in enumerate(scenes):
         p64 = ctx.range_compress(s.ph.to(torch.complex128), s.nfft)
@@ -676,6 +681,7 @@ def serving_leg(ctx, torch, n_scenes: int = 64, pga_iters: int = 5, n_streams: i
         "batch_rc_bp_gpu_ms": round(t_rcbp, 3),
         "batch_Gpixel_pulses_per_s": round(total_pp / (t_batch * 1e-3) / 1e9, 1),
         "batch_rc_bp_Gpixel_pulses_per_s": round(total_pp / (t_rcbp * 1e-3) / 1e9, 1),
+        "scenes_tile": scene_tile,
         "pga_streams": n_streams,
         "batch_graph_replay_gpu_ms": None if t_graph is None else round(t_graph, 3),
         "bp_rel_l2_vs_fp64_max": max(rels),

@@ -261,6 +261,14 @@ int sarf_bp_scenes(sarf_ctx *ctx, int nscenes,
                    const sarf_scene *scenes, /* host array of nscenes descriptors */
                    int tile, void *stream);
 
+/* Tile tuner for sarf_bp_scenes: times the batch with every tile 1..5 that  */
+/* meets the accuracy bound of every scene (`reps` runs each, after one      */
+/* warm run), waits for the stream, and returns the fastest in *best_tile    */
+/* (and its mean time in *best_ms, may be NULL).  Images hold the last       */
+/* run's result.  No reference counterpart (see sarf_autotune).              */
+int sarf_autotune_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scenes, int reps,
+                         int *best_tile, float *best_ms, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Phase gradient autofocus of a focused image, in place (matlab/PGA.m:     */
 /* 151-232; C++ restatement src/autofocus/cpu/autofocus_cpu.cpp:249-474).   */

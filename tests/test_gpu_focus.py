@@ -258,6 +258,12 @@ def test_bp_scenes_uneven_batch_matches_single_calls(ctx):
             assert oracle.rel_l2(img, ref) <= 1e-4
             if tile == 0:
                 assert oracle.rel_l2(img, single) < 1e-5
+    # the batch tuner returns an accurate tile whose batch meets the same bound
+    best, ms = ctx.autotune_scenes(scenes, reps=1)
+    assert 1 <= best <= 5 and ms > 0
+    imgs = ctx.backproject_scenes(scenes, tile=best)
+    for img, (single, ref) in zip(imgs, refs):
+        assert oracle.rel_l2(img.cpu().numpy(), ref) <= 1e-4
 
 
 @pytest.mark.parametrize("K,Np,S", [(300, 17, 2000), (64, 9, 500), (1025, 5, 3000)])

@@ -81,6 +81,7 @@ SIGNATURES = {
     "sarf_autotune": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp]),
     "sarf_matched_filter": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sarf_image_db8": (_i, [_vp, _vp, _i, _i, C.c_float, _vp, _vp]),
+    "sarf_autotune_scenes": (_i, [_vp, _i, C.c_void_p, _i, _vp, _vp, _vp]),
 }
 
 

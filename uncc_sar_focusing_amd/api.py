@@ -249,6 +249,20 @@ class Context:
         """Backproject many independent scenes in one persistent launch.
         `scenes` is a sequence of (prof, ant, r0, f0, grid, maxWr) or
         (prof, ant, r0, f0, grid, maxWr, img); returns the images."""
+        descs, keep = self._scene_descs(scenes)
+        check(self.lib.sarf_bp_scenes(self._h, len(scenes), descs, tile, self._stream()))
+        return keep
+
+    def autotune_scenes(self, scenes, reps: int = 2) -> tuple[int, float]:
+        """Fastest accurate batch tile for backproject_scenes on these scenes
+        (sarf_autotune_scenes): returns (tile, mean ms)."""
+        descs, _ = self._scene_descs(scenes)
+        tile, ms = C.c_int(0), C.c_float(0.0)
+        check(self.lib.sarf_autotune_scenes(self._h, len(scenes), descs, reps, C.byref(tile),
+                                            C.byref(ms), self._stream()))
+        return tile.value, ms.value
+
+    def _scene_descs(self, scenes):
         descs = (_lib.SceneDesc * len(scenes))()
         keep = []
         for i, sc in enumerate(scenes):
@@ -265,8 +279,7 @@ class Context:
                                       f0.data_ptr(), img.data_ptr(), grid.Nx, grid.Ny, nfft, Np,
                                       grid.x0, grid.y0, grid.Wx, grid.Wy, maxWr, 0.0, 0.0)
             keep.append(img)
-        check(self.lib.sarf_bp_scenes(self._h, len(scenes), descs, tile, self._stream()))
-        return keep
+        return descs, keep
 
     # ----------------------------------------------------------------- PGA
     def pga(self, img: torch.Tensor, iters: int, db_down: float = 30.0):

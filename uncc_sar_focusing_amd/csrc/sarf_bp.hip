@@ -158,6 +158,22 @@ int bp_pick_tile(const sarf_bp_desc *d, double dx, double dy) {
     return 1;
 }
 
+int bp_tile_shape(int id, int *tx, int *ty);
+
+// Whether tile `id` meets the default choice's accuracy bound (LOS-projected
+// half extent) and fits its worst-case profile window in LDS.
+bool bp_tile_accurate(const sarf_bp_desc *d, int id) {
+    int tx = 0, ty = 0;
+    if (!bp_tile_shape(id, &tx, &ty) || d->Nx < 2 || d->Ny < 2 || !(d->maxWr > 0)) return false;
+    const double dx = d->Wx / (d->Nx - 1), dy = d->Wy / (d->Ny - 1);
+    const double invbin = d->Nfft / d->maxWr;
+    const bool hint = (d->los_x != 0.0 || d->los_y != 0.0);
+    const double lx = hint ? std::fabs(d->los_x) : 1.0, ly = hint ? std::fabs(d->los_y) : 1.0;
+    const double bound = hint ? 2.6 : 3.2;
+    const double hx = 0.5 * (tx - 1) * dx, hy = 0.5 * (ty - 1) * dy;
+    return lx * hx + ly * hy <= bound && 2.0 * (hx + hy) * invbin + 8 <= kBpCap;
+}
+
 // Returns the tile id if known (and its TX, TY), else 0.
 int bp_tile_shape(int id, int *tx, int *ty) {
     for (const TileShape &t : kTiles)
@@ -332,20 +348,9 @@ namespace {
 
 // Tiles that meet the default choice's accuracy bound (and the LDS window).
 std::vector<int> tune_tiles(const sarf_bp_desc *d) {
-    const double dx = d->Wx / (d->Nx - 1), dy = d->Wy / (d->Ny - 1);
-    const double invbin = d->Nfft / d->maxWr;
-    const bool hint = (d->los_x != 0.0 || d->los_y != 0.0);
-    const double lx = hint ? std::fabs(d->los_x) : 1.0, ly = hint ? std::fabs(d->los_y) : 1.0;
-    const double bound = hint ? 2.6 : 3.2;
     std::vector<int> out;
-    for (int id = 1; id <= bp_tile_count(); id++) {
-        int tx = 0, ty = 0;
-        if (!bp_tile_shape(id, &tx, &ty)) continue;
-        const double hx = 0.5 * (tx - 1) * dx, hy = 0.5 * (ty - 1) * dy;
-        if (lx * hx + ly * hy > bound) continue;
-        if (2.0 * (hx + hy) * invbin + 8 > kBpCap) continue;
-        out.push_back(id);
-    }
+    for (int id = 1; id <= bp_tile_count(); id++)
+        if (bp_tile_accurate(d, id)) out.push_back(id);
     return out;
 }
 

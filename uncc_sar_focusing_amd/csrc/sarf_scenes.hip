@@ -21,6 +21,7 @@ namespace sarf {
 int bp_pick_tile(const sarf_bp_desc *d, double dx, double dy);
 int bp_tile_shape(int id, int *tx, int *ty);
 int bp_fill_args(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, BpArgs *a);
+bool bp_tile_accurate(const sarf_bp_desc *d, int id);
 
 namespace {
 
@@ -75,6 +76,35 @@ void launch_scenes_t(int grid, const SceneDev *sc, const int2 *items, int nitems
 
 using namespace sarf;
 
+namespace sarf {
+namespace {
+
+// Default batch tile: the choice for the finest-spaced scene (it satisfies
+// the precision bound for all), at most 8 pixels per lane.
+int scenes_default_tile(int nscenes, const sarf_scene *scenes) {
+    int tid = 1;
+    double best_sp = 1e300;
+    for (int i = 0; i < nscenes; i++) {
+        const sarf_scene &c = scenes[i];
+        if (c.Nx < 2 || c.Ny < 2) continue;
+        const double sp = std::max(c.Wx / (c.Nx - 1), c.Wy / (c.Ny - 1));
+        if (sp < best_sp) {
+            best_sp = sp;
+            sarf_bp_desc d{};
+            d.Nx = c.Nx; d.Ny = c.Ny; d.Nfft = c.Nfft; d.Np = c.Np;
+            d.x0 = c.x0; d.y0 = c.y0; d.Wx = c.Wx; d.Wy = c.Wy; d.maxWr = c.maxWr;
+            d.los_x = c.los_x; d.los_y = c.los_y;
+            tid = bp_pick_tile(&d, c.Wx / (c.Nx - 1), c.Wy / (c.Ny - 1));
+        }
+    }
+    if (tid == 6) tid = 3;
+    if (tid == 7) tid = 4;
+    return tid;
+}
+
+}  // namespace
+}  // namespace sarf
+
 extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scenes, int tile,
                               void *stream) {
     clear_error();
@@ -86,28 +116,8 @@ extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scen
         double cost;
     };
     std::vector<Item> items;
-    // one tile shape for the whole batch: the given one, else the choice for
-    // the finest-spaced scene (it satisfies the precision bound for all)
-    int tid = tile;
-    if (tid <= 0) {
-        double best_sp = 1e300;
-        for (int i = 0; i < nscenes; i++) {
-            const sarf_scene &c = scenes[i];
-            if (c.Nx < 2 || c.Ny < 2) continue;
-            const double sp = std::max(c.Wx / (c.Nx - 1), c.Wy / (c.Ny - 1));
-            if (sp < best_sp) {
-                best_sp = sp;
-                sarf_bp_desc d{};
-                d.Nx = c.Nx; d.Ny = c.Ny; d.Nfft = c.Nfft; d.Np = c.Np;
-                d.x0 = c.x0; d.y0 = c.y0; d.Wx = c.Wx; d.Wy = c.Wy; d.maxWr = c.maxWr;
-                d.los_x = c.los_x; d.los_y = c.los_y;
-                tid = bp_pick_tile(&d, c.Wx / (c.Nx - 1), c.Wy / (c.Ny - 1));
-            }
-        }
-        // the batch kernel keeps 8 pixels per lane at most
-        if (tid == 6) tid = 3;
-        if (tid == 7) tid = 4;
-    }
+    // one tile shape for the whole batch: the given one, else the default
+    int tid = tile > 0 ? tile : scenes_default_tile(nscenes, scenes);
     int tx = 0, ty = 0;
     if (!bp_tile_shape(tid, &tx, &ty) || tid > 5)
         return fail(SARF_ERR_INVALID, "sarf_bp_scenes: tile %d not available for batches", tid);
@@ -177,5 +187,66 @@ extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scen
         case 5: launch_scenes_t<16, 4>(grid, dsc, ditems, nit, counter, s); break;
     }
     SARF_HIP(hipGetLastError());
+    return SARF_OK;
+}
+
+// Times sarf_bp_scenes with the default batch tile and every other tile
+// (1..5) that meets the accuracy bound for all scenes, and returns the
+// fastest (1 % hysteresis towards the default, then the smaller id); waits
+// for the stream.  The images hold the last run's result.
+extern "C" int sarf_autotune_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scenes, int reps,
+                                    int *best_tile, float *best_ms, void *stream) {
+    clear_error();
+    SARF_TRY(use_device(ctx));
+    if (nscenes < 1 || !scenes || !best_tile || reps < 1 || reps > 100)
+        return fail(SARF_ERR_INVALID, "sarf_autotune_scenes: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SARF_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(SARF_ERR_HIP, "sarf_autotune_scenes: event creation failed");
+    }
+    const int def = scenes_default_tile(nscenes, scenes);
+    std::vector<int> cand{def};
+    for (int id = 1; id <= 5; id++) {
+        if (id == def) continue;
+        bool ok = true;
+        for (int i = 0; i < nscenes && ok; i++) {
+            const sarf_scene &c = scenes[i];
+            sarf_bp_desc d{};
+            d.Nx = c.Nx; d.Ny = c.Ny; d.Nfft = c.Nfft; d.Np = c.Np;
+            d.x0 = c.x0; d.y0 = c.y0; d.Wx = c.Wx; d.Wy = c.Wy; d.maxWr = c.maxWr;
+            d.los_x = c.los_x; d.los_y = c.los_y;
+            ok = bp_tile_accurate(&d, id);
+        }
+        if (ok) cand.push_back(id);
+    }
+    int best = 0;
+    float bms = 0.f;
+    int rc = SARF_OK;
+    for (int id : cand) {
+        rc = sarf_bp_scenes(ctx, nscenes, scenes, id, stream);  // upload + warm
+        if (rc != SARF_OK) break;
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(SARF_ERR_HIP, "event record"); break; }
+        for (int r = 0; r < reps && rc == SARF_OK; r++) rc = sarf_bp_scenes(ctx, nscenes, scenes, id, stream);
+        if (rc != SARF_OK) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(SARF_ERR_HIP, "sarf_autotune_scenes: timing failed");
+            break;
+        }
+        ms /= reps;
+        if (best == 0 || ms < 0.99f * bms) {
+            best = id;
+            bms = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != SARF_OK) return rc;
+    *best_tile = best;
+    if (best_ms) *best_ms = bms;
     return SARF_OK;
 }
