@@ -153,11 +153,10 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
                         const float g = fmaf(s, A.w, A.z);
                         const float dl = fmaf(tq, g, -s);
                         const float p = fmaf(dl, invbin, B.x);
-                        const float fr = __builtin_amdgcn_fractf(p);
                         const int i = (int)__builtin_amdgcn_fmed3f(p, 0.f, wmax);
                         const float4 w = L.win[I.y + i];
-                        const float vr = fmaf(fr, w.z, w.x);
-                        const float vi = fmaf(fr, w.w, w.y);
+                        const float vr = fmaf(p, w.z, w.x);
+                        const float vi = fmaf(p, w.w, w.y);
                         const float ph = fmaf(dl, B.z, B.y);
                         const float sn = __builtin_amdgcn_sinf(ph);
                         const float cs = __builtin_amdgcn_cosf(ph);

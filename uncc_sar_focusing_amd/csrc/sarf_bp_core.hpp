@@ -25,8 +25,12 @@
 //         dR - dR_c = -s + t (g0 + g1 s),  g0 = 1/(2 R_c), g1 = 1/(2 R_c^2)
 //     (expansion of (R_c - s) sqrt(1 + t/(R_c - s)^2) - R_c; remainder
 //     t^2/(8 R^3)), so no square root per pixel.
-//   * Profile windows are staged in LDS as {s_i, s_{i+1} - s_i} so the lerp
-//     is one 16-byte LDS read and two FMAs; a chunk's windows are packed.
+//   * Profile windows are staged in LDS as intercept and slope in
+//     window-local bin units, {s_i - i d_i, d_i} with d_i = s_{i+1} - s_i,
+//     so the lerp at position p (i = floor p) is one 16-byte LDS read and
+//     two FMAs on p itself: no fract.  The intercept's rounding is of the
+//     order of ulp(p) |d_i|, the error p itself already carries.  A chunk's
+//     windows are packed.
 //   * Phase psi = psi_c + (2 f0/c)(dR - dR_c) in revolutions goes straight
 //     into v_sin_f32 / v_cos_f32 (they take revolutions and reduce exactly
 //     for |psi| <= 256).
@@ -183,7 +187,9 @@ struct BpTile {
                 float2 s0 = make_float2(0.f, 0.f), s1 = make_float2(0.f, 0.f);
                 if (g >= 0 && g < a.Nfft) s0 = src[g];
                 if (g + 1 >= 0 && g + 1 < a.Nfft) s1 = src[g + 1];
-                L.win[I.y + b] = make_float4(s0.x, s0.y, s1.x - s0.x, s1.y - s0.y);
+                const float dr = s1.x - s0.x, di = s1.y - s0.y;
+                const float fb = (float)b;
+                L.win[I.y + b] = make_float4(fmaf(-fb, dr, s0.x), fmaf(-fb, di, s0.y), dr, di);
             }
         }
         __syncthreads();
@@ -211,11 +217,10 @@ struct BpTile {
                     const float g = fmaf(s, A.w, A.z);
                     const float dl = fmaf(tq, g, -s);  // dR - dR_c
                     const float p = fmaf(dl, invbin, B.x);
-                    const float fr = __builtin_amdgcn_fractf(p);
                     const int i = (int)p;
                     const float4 w = L.win[I.y + i];
-                    const float vr = fmaf(fr, w.z, w.x);
-                    const float vi = fmaf(fr, w.w, w.y);
+                    const float vr = fmaf(p, w.z, w.x);
+                    const float vi = fmaf(p, w.w, w.y);
                     const float ph = fmaf(dl, B.z, B.y);  // revolutions
                     const float sn = __builtin_amdgcn_sinf(ph);
                     const float cs = __builtin_amdgcn_cosf(ph);
@@ -233,12 +238,11 @@ struct BpTile {
                     const float g = fmaf(s, A.w, A.z);
                     const float dl = fmaf(tq, g, -s);
                     const float p = fmaf(dl, invbin, B.x);
-                    const float fr = __builtin_amdgcn_fractf(p);
                     const int i = (int)__builtin_amdgcn_fmed3f(p, 0.f, wmax);
                     const float4 w = L.win[I.y + i];
                     const bool ok = p > glo && p < ghi;
-                    const float vr = ok ? fmaf(fr, w.z, w.x) : 0.f;
-                    const float vi = ok ? fmaf(fr, w.w, w.y) : 0.f;
+                    const float vr = ok ? fmaf(p, w.z, w.x) : 0.f;
+                    const float vi = ok ? fmaf(p, w.w, w.y) : 0.f;
                     const float ph = fmaf(dl, B.z, B.y);
                     const float sn = __builtin_amdgcn_sinf(ph);
                     const float cs = __builtin_amdgcn_cosf(ph);
