@@ -157,7 +157,8 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
                         const float4 w = L.win[I.y + i];
                         const float vr = fmaf(p, w.z, w.x);
                         const float vi = fmaf(p, w.w, w.y);
-                        const float ph = fmaf(dl, B.z, B.y);
+                        // one revolution: v_sin/v_cos take |x| <= 256
+                        const float ph = __builtin_amdgcn_fractf(fmaf(dl, B.z, B.y));
                         const float sn = __builtin_amdgcn_sinf(ph);
                         const float cs = __builtin_amdgcn_cosf(ph);
                         // z = v' + j k v

@@ -33,8 +33,11 @@
 //     windows are packed.
 //   * Phase psi = psi_c + (2 f0/c)(dR - dR_c) in revolutions goes straight
 //     into v_sin_f32 / v_cos_f32 (they take revolutions and reduce exactly
-//     for |psi| <= 256).
-//   * Interior tiles skip the per-pixel swath gate (uniform per pulse).
+//     for |psi| <= 256; outside that range they are not valid).
+//   * Fast pulses (uniform per (tile, pulse)): the window lies inside the
+//     profile, so no per-pixel swath gate, and (2 f0/c)|dR - dR_c| <= 250
+//     over the tile, so no phase reduction.  Other pulses take the gated
+//     path, which also reduces the phase to one revolution.
 #pragma once
 
 #include "sarf_internal.hpp"
@@ -162,7 +165,10 @@ struct BpTile {
                                     (float)(0.5 * iR * iR));
             L.pB[tid] = make_float4((float)(posg - (double)start), (float)psi, (float)kap,
                                     (float)(W - 2));
-            const int interior = (start >= 0 && start + W <= a.Nfft) ? 1 : 0;
+            // Fast path: the window lies inside the profile (no swath gate) and
+            // |psi| <= 0.5 + kap * ext stays inside v_sin/v_cos's input range
+            // (|x| <= 256 revolutions), so the phase needs no reduction.
+            const int interior = (start >= 0 && start + W <= a.Nfft && kap * ext <= 250.0) ? 1 : 0;
             L.pI[tid] = make_int4((int)start, 0, W, interior);
         }
         __syncthreads();
@@ -228,7 +234,9 @@ struct BpTile {
                     acci[j] = fmaf(vr, sn, fmaf(vi, cs, acci[j]));
                 }
             } else {
-                // swath gate r_0 < dR < r_{N-1}  <=>  0 < pos_global < N - 1
+                // swath gate r_0 < dR < r_{N-1}  <=>  0 < pos_global < N - 1;
+                // the phase is reduced to one revolution (high carriers or
+                // large tiles take it past the sine unit's range)
                 const float glo = (float)(-I.x);
                 const float ghi = (float)(a.Nfft - 1 - I.x);
 #pragma unroll
@@ -243,7 +251,7 @@ struct BpTile {
                     const bool ok = p > glo && p < ghi;
                     const float vr = ok ? fmaf(p, w.z, w.x) : 0.f;
                     const float vi = ok ? fmaf(p, w.w, w.y) : 0.f;
-                    const float ph = fmaf(dl, B.z, B.y);
+                    const float ph = __builtin_amdgcn_fractf(fmaf(dl, B.z, B.y));
                     const float sn = __builtin_amdgcn_sinf(ph);
                     const float cs = __builtin_amdgcn_cosf(ph);
                     accr[j] = fmaf(vr, cs, fmaf(-vi, sn, accr[j]));
