@@ -19,6 +19,8 @@
 // the first pass is bank-conflict free.  N up to 16384 fits (139 KiB of the
 // 160 KiB LDS).  The FP64 variant (validation builds) uses a global-memory
 // radix-4 Stockham since 16384 double2 do not fit in LDS.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 
 #include "sarf_internal.hpp"
@@ -132,7 +134,12 @@ constexpr int rc_threads(int log2n) {
     return ((1 << log2n) / 16 >= 256) ? (1 << log2n) / 16 : 256;
 }
 
-template <int LOG2N>
+// ROWS: how many of a lane's 16 sample rows k = j + i*T (i < 16) can hold
+// samples, i.e. ceil(K/T) rounded up to 4 or 16.  With ROWS = 4 only four
+// values per lane are loaded and the next pulse group's are prefetched into
+// registers; with ROWS = 16 (K > N/4) the group's samples are loaded at the
+// top of its iteration.
+template <int LOG2N, int ROWS>
 __global__ __launch_bounds__(rc_threads(LOG2N)) void rc_lds_f32(
     const float2 *__restrict__ ph, int K, int Np, int flags, const float *__restrict__ taper,
     const double *__restrict__ f0, const double *__restrict__ r0,
@@ -146,79 +153,128 @@ __global__ __launch_bounds__(rc_threads(LOG2N)) void rc_lds_f32(
     constexpr int PADN = N + N / 16;
     __shared__ float2 lds[P * PADN];
 
+    const int ngroups = (Np + P - 1) / P;
     const int tid = threadIdx.x;
     const int lp = tid / T, j = tid % T;
-    const int n = blockIdx.x * P + lp;
-    const bool active = n < Np;
     float2 *L = lds + lp * PADN;
 
-    float2 v[16];
-    // ---- pass 0: global -> LDS, radix R0, Ns = 1 (no twiddles) ----------
-    {
-        const float2 *src = ph + (size_t)(active ? n : 0) * K;
+    // Sample row i of pulse n (sample k = j + i*T) as pass 0 consumes it:
+    // zero padding (k >= K is never loaded), taper, fftshift fold (-1)^k.
+    auto sample = [&](int n, int i, int jl) {
+        const int k = jl + i * T;
+        float2 v = make_float2(0.f, 0.f);
+        if (n < Np && k < K) {
+            v = ph[(size_t)n * K + k];
+            if (flags & SARF_RC_TAPER) {
+                const float w = taper[k];
+                v.x *= w;
+                v.y *= w;
+            }
+            if (k & 1) v = make_float2(-v.x, -v.y);
+        }
+        return v;
+    };
+    // Raw prefetch of group g's rows (no use of the values here, so the
+    // loads stay in flight): the address is clamped to sample 0 for rows past
+    // K or pulses past Np, and pass 0 masks those rows to zero.
+    auto fetch = [&](int g, float2 *x, float *w, int jl) {
+        const int n = g * P + lp;
+#pragma unroll
+        for (int i = 0; i < ROWS; i++) {
+            const int k = jl + i * T;
+            const bool ok = n < Np && k < K;
+            x[i] = ph[ok ? (size_t)n * K + k : 0];
+            w[i] = (flags & SARF_RC_TAPER) ? taper[ok ? k : 0] : 1.f;
+        }
+    };
+
+    // Persistent over pulse groups (the grid is at most one resident wave of
+    // workgroups).  With ROWS = 4 the next group's samples are loaded into
+    // registers while this group's LDS passes run, and this group's stores
+    // drain while the next group computes, so HBM traffic overlaps the FFT.
+    constexpr bool kPrefetch = ROWS <= 4;
+    float2 xin[kPrefetch ? ROWS : 1];
+    float win[kPrefetch ? ROWS : 1];
+    if (kPrefetch) fetch(blockIdx.x, xin, win, j);
+    for (int g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        const int n = g * P + lp;
+        const bool active = n < Np;
+        // Opaque copy of the lane index: keeps LLVM from hoisting every
+        // pass's LDS addresses and twiddle indices out of the pulse loop,
+        // which would hold them in registers across iterations.
+        int jj = j;
+        __asm__ volatile("" : "+v"(jj));
+        float2 v[16];
+        // ---- pass 0: registers -> LDS, radix R0, Ns = 1 (no twiddles) ------
+        // lane input r of butterfly b is sample row i = b + r * NB0
 #pragma unroll
         for (int b = 0; b < NB0; b++) {
-            const int jb = j + b * T;
+            const int jb = jj + b * T;
 #pragma unroll
             for (int r = 0; r < R0; r++) {
-                const int k = jb + r * (N / R0);
-                float2 x = make_float2(0.f, 0.f);
-                if (active && k < K) {
-                    x = src[k];
-                    if (flags & SARF_RC_TAPER) {
-                        const float w = taper[k];
-                        x.x *= w;
-                        x.y *= w;
+                const int i = b + r * NB0;
+                if (kPrefetch) {
+                    float2 x = make_float2(0.f, 0.f);
+                    const int k = jj + i * T;
+                    if (i < ROWS && active && k < K) {
+                        const int ic = i < ROWS ? i : 0;
+                        x = xin[ic];
+                        x.x *= win[ic];
+                        x.y *= win[ic];
+                        if (k & 1) x = make_float2(-x.x, -x.y);
                     }
-                    if (k & 1) x = make_float2(-x.x, -x.y);  // fftshift fold
+                    v[b * R0 + r] = x;
+                } else {
+                    v[b * R0 + r] = sample(n, i, jj);
                 }
-                v[b * R0 + r] = x;
             }
             dft<+1, R0>(v + b * R0);
 #pragma unroll
             for (int r = 0; r < R0; r++) L[lpad(jb * R0 + r)] = v[b * R0 + r];
         }
-    }
-    __syncthreads();
+        if (kPrefetch && g + (int)gridDim.x < ngroups) fetch(g + (int)gridDim.x, xin, win, jj);
+        __syncthreads();
 
-    // ---- radix-16 passes --------------------------------------------------
-    constexpr int NPASS16 = (LOG2N - (B == 0 ? 4 : B)) / 4;
-    int Ns = R0;
+        // ---- radix-16 passes ----------------------------------------------
+        constexpr int NPASS16 = (LOG2N - (B == 0 ? 4 : B)) / 4;
+        int Ns = R0;
 #pragma unroll
-    for (int pass = 0; pass < NPASS16; pass++) {
-        const bool last = (pass == NPASS16 - 1);
+        for (int pass = 0; pass < NPASS16; pass++) {
+            const bool last = (pass == NPASS16 - 1);
 #pragma unroll
-        for (int r = 0; r < 16; r++) v[r] = L[lpad(j + r * (N / 16))];
-        const int k = j & (Ns - 1);
-        if (k != 0) {
-            float2 p[16];
-            powers16(tw[k * (N / (Ns * 16))], p);
+            for (int r = 0; r < 16; r++) v[r] = L[lpad(jj + r * (N / 16))];
+            const int k = jj & (Ns - 1);
+            if (k != 0) {
+                float2 p[16];
+                powers16(tw[k * (N / (Ns * 16))], p);
 #pragma unroll
-            for (int r = 1; r < 16; r++) v[r] = cmul(v[r], p[r]);
-        }
-        dft<+1, 16>(v);
-        if (!last) {
-            __syncthreads();  // every lane has read before anyone overwrites
-            const int base = (j - k) * 16 + k;
-#pragma unroll
-            for (int r = 0; r < 16; r++) L[lpad(base + r * Ns)] = v[r];
-            __syncthreads();
-        } else if (active) {
-            // Ns = N/16: output index j + r*N/16, coalesced per r
-            float sr = 1.0f / N, si = 0.0f;
-            if (flags & SARF_RC_RAMP) {
-                double a = -4.0 * kPi * f0[n] * r0[n] / kClight;
-                sr = (float)(cos(a) / N);
-                si = (float)(sin(a) / N);
+                for (int r = 1; r < 16; r++) v[r] = cmul(v[r], p[r]);
             }
-            float2 *dst = prof + (size_t)n * N;
+            dft<+1, 16>(v);
+            if (!last) {
+                __syncthreads();  // every lane has read before anyone overwrites
+                const int base = (jj - k) * 16 + k;
 #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const float2 x = v[r];
-                dst[j + r * (N / 16)] = make_float2(x.x * sr - x.y * si, x.x * si + x.y * sr);
+                for (int r = 0; r < 16; r++) L[lpad(base + r * Ns)] = v[r];
+                __syncthreads();
+            } else if (active) {
+                // Ns = N/16: output index j + r*N/16, coalesced per r
+                float sr = 1.0f / N, si = 0.0f;
+                if (flags & SARF_RC_RAMP) {
+                    double a = -4.0 * kPi * f0[n] * r0[n] / kClight;
+                    sr = (float)(cos(a) / N);
+                    si = (float)(sin(a) / N);
+                }
+                float2 *dst = prof + (size_t)n * N;
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const float2 x = v[r];
+                    dst[jj + r * (N / 16)] = make_float2(x.x * sr - x.y * si, x.x * si + x.y * sr);
+                }
             }
+            Ns *= 16;
         }
-        Ns *= 16;
+        __syncthreads();  // the last pass has read LDS before the next pass 0 writes
     }
 }
 
@@ -354,17 +410,39 @@ __global__ __launch_bounds__(256) void rc_direct(const T2 *__restrict__ ph, int 
     }
 }
 
-template <int LOG2N>
-int launch_lds(const sarf_rc_desc *d, const float2 *ph, const float *taper, const double *f0,
-               const double *r0, const float2 *tw, float2 *prof, hipStream_t s) {
+template <int LOG2N, int ROWS>
+int launch_lds_rows(const sarf_ctx *ctx, const sarf_rc_desc *d, const float2 *ph, const float *taper,
+               const double *f0, const double *r0, const float2 *tw, float2 *prof,
+               hipStream_t s) {
     constexpr int N = 1 << LOG2N;
     constexpr int T = N / 16;
     constexpr int P = T >= 256 ? 1 : 256 / T;
-    const int grid = (d->Np + P - 1) / P;
-    hipLaunchKernelGGL(rc_lds_f32<LOG2N>, dim3(grid), dim3(T * P), 0, s, ph, d->K, d->Np,
+    const int ngroups = (d->Np + P - 1) / P;
+    // one resident wave of workgroups, each looping over pulse groups
+    static std::atomic<int> per_cu{0};
+    int occ = per_cu.load(std::memory_order_relaxed);
+    if (occ == 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, rc_lds_f32<LOG2N, ROWS>, T * P, 0) !=
+                hipSuccess ||
+            occ < 1)
+            occ = 1;
+        per_cu.store(occ, std::memory_order_relaxed);
+    }
+    const int cus = ctx->cu_count > 0 ? ctx->cu_count : 256;
+    const int grid = std::min(ngroups, occ * cus);
+    hipLaunchKernelGGL((rc_lds_f32<LOG2N, ROWS>), dim3(grid), dim3(T * P), 0, s, ph, d->K, d->Np,
                        d->flags, taper, f0, r0, tw, prof);
     SARF_HIP(hipGetLastError());
     return SARF_OK;
+}
+
+template <int LOG2N>
+int launch_lds(const sarf_ctx *ctx, const sarf_rc_desc *d, const float2 *ph, const float *taper,
+               const double *f0, const double *r0, const float2 *tw, float2 *prof,
+               hipStream_t s) {
+    constexpr int T = (1 << LOG2N) / 16;
+    if (d->K <= 4 * T) return launch_lds_rows<LOG2N, 4>(ctx, d, ph, taper, f0, r0, tw, prof, s);
+    return launch_lds_rows<LOG2N, 16>(ctx, d, ph, taper, f0, r0, tw, prof, s);
 }
 
 }  // namespace
@@ -447,16 +525,16 @@ int launch_range_compress(sarf_ctx *ctx, const sarf_rc_desc *d, const void *ph,
     const float2 *x = (const float2 *)ph;
     float2 *y = (float2 *)prof;
     switch (log2n) {
-        case 5: return launch_lds<5>(d, x, taper, f0, r0, tw, y, s);
-        case 6: return launch_lds<6>(d, x, taper, f0, r0, tw, y, s);
-        case 7: return launch_lds<7>(d, x, taper, f0, r0, tw, y, s);
-        case 8: return launch_lds<8>(d, x, taper, f0, r0, tw, y, s);
-        case 9: return launch_lds<9>(d, x, taper, f0, r0, tw, y, s);
-        case 10: return launch_lds<10>(d, x, taper, f0, r0, tw, y, s);
-        case 11: return launch_lds<11>(d, x, taper, f0, r0, tw, y, s);
-        case 12: return launch_lds<12>(d, x, taper, f0, r0, tw, y, s);
-        case 13: return launch_lds<13>(d, x, taper, f0, r0, tw, y, s);
-        case 14: return launch_lds<14>(d, x, taper, f0, r0, tw, y, s);
+        case 5: return launch_lds<5>(ctx, d, x, taper, f0, r0, tw, y, s);
+        case 6: return launch_lds<6>(ctx, d, x, taper, f0, r0, tw, y, s);
+        case 7: return launch_lds<7>(ctx, d, x, taper, f0, r0, tw, y, s);
+        case 8: return launch_lds<8>(ctx, d, x, taper, f0, r0, tw, y, s);
+        case 9: return launch_lds<9>(ctx, d, x, taper, f0, r0, tw, y, s);
+        case 10: return launch_lds<10>(ctx, d, x, taper, f0, r0, tw, y, s);
+        case 11: return launch_lds<11>(ctx, d, x, taper, f0, r0, tw, y, s);
+        case 12: return launch_lds<12>(ctx, d, x, taper, f0, r0, tw, y, s);
+        case 13: return launch_lds<13>(ctx, d, x, taper, f0, r0, tw, y, s);
+        case 14: return launch_lds<14>(ctx, d, x, taper, f0, r0, tw, y, s);
     }
     return fail(SARF_ERR_UNSUPPORTED, "range_compress: Nfft=%d", d->Nfft);
 }
