@@ -566,8 +566,8 @@ def serving_leg(ctx, torch, n_scenes: int = 64, pga_iters: int = 5, n_streams: i
              for s in scenes]
     imgs = [torch.empty((s.grid.Nx, s.grid.Ny), dtype=torch.complex64, device="cuda:0")
             for s in scenes]
-    descs = [(profs[i], s.ant, s.r0, s.f0, s.grid, s.geom.maxWr, imgs[i])
-             for i, s in enumerate(scenes)]
+    descs = [(profs[i], s.ant, s.r0, s.f0, s.grid, s.geom.maxWr, imgs[i],
+              s.geom.los(s.grid.x0, s.grid.y0)) for i, s in enumerate(scenes)]
 
     # batch tile for the one-launch backprojection of all scenes
     for s, p in zip(scenes, profs):

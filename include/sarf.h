@@ -112,7 +112,9 @@ typedef struct {
     double maxWr;   /* unambiguous range c/(2 mean deltaF) = max_Wy_m */
     int precision;  /* SARF_BP_MIXED | SARF_BP_F64 */
     int flags;      /* SARF_BP_ACCUMULATE */
-    int tile;       /* 0 = default for the pixel spacing; else a tile id 1..7 */
+    int tile;       /* 0 = default for the pixel spacing and line of sight; else a
+                       tile id: 1..7 (lanes along y) or 11..14 (lanes along x);
+                       batches take 1..5 and 11..14, the gradient 1..5 */
     int pulse_split;/* 0 = automatic; n >= 1: pulses in n contiguous groups, each
                        forming a partial image, summed in group order */
     double los_x, los_y; /* optional hint: horizontal components of the unit line
@@ -237,7 +239,7 @@ int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *desc,
 /* persistent launch (work items = scene tiles, largest first, drained      */
 /* through an atomic counter).  Each scene is described as for              */
 /* sarf_backproject, with its own device buffers; images are overwritten.   */
-/* tile: 0 = chosen for the finest-spaced scene, else a tile id 1..5.       */
+/* tile: 0 = chosen for the finest-spaced scene, else 1..5 or 11..14.      */
 /* The work list is uploaded (and the call waits for the copy) only when    */
 /* the descriptors or tile differ from the previous call on this context;   */
 /* a repeated call only enqueues kernels, so after one call outside a       */
@@ -261,7 +263,7 @@ int sarf_bp_scenes(sarf_ctx *ctx, int nscenes,
                    const sarf_scene *scenes, /* host array of nscenes descriptors */
                    int tile, void *stream);
 
-/* Tile tuner for sarf_bp_scenes: times the batch with every tile 1..5 that  */
+/* Tile tuner for sarf_bp_scenes: times the batch with every batch tile that*/
 /* meets the accuracy bound of every scene (`reps` runs each, after one      */
 /* warm run), waits for the stream, and returns the fastest in *best_tile    */
 /* (and its mean time in *best_ms, may be NULL).  Images hold the last       */

@@ -41,7 +41,9 @@ def _los_scene(Np=48, K=64, n=40, spacing=0.12, true_coef=(0.02, -0.015, 0.01)):
     return geom, ph, grid
 
 
-def test_batch_entropy_matches_oracle_per_candidate(ctx):
+@pytest.mark.parametrize("tile", [0, 4, 11, 12, 13, 14])
+def test_batch_entropy_matches_oracle_per_candidate(ctx, tile):
+    """Every batch tile shape, both lane orientations (0 = default choice)."""
     geom, ph, grid = _los_scene()
     nfft = 512
     basis = af.poly_basis(geom.Np)
@@ -50,7 +52,8 @@ def test_batch_entropy_matches_oracle_per_candidate(ctx):
     prof_h = oracle.range_compress_f64(ph, nfft)
     prof = to_dev(prof_h.astype(np.complex64))
     ent = ctx.batch_entropy(prof, to_dev(geom.ant), to_dev(geom.r0), to_dev(geom.f0), grid,
-                            geom.maxWr, to_dev(coefs), to_dev(basis), to_dev(direction)).cpu().numpy()
+                            geom.maxWr, to_dev(coefs), to_dev(basis), to_dev(direction),
+                            tile=tile).cpu().numpy()
     ref = []
     for c in coefs:
         ant = af.apply_los_offsets(geom.ant, c, basis, direction)

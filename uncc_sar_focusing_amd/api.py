@@ -247,8 +247,9 @@ class Context:
 
     def backproject_scenes(self, scenes, *, tile: int = 0) -> list:
         """Backproject many independent scenes in one persistent launch.
-        `scenes` is a sequence of (prof, ant, r0, f0, grid, maxWr) or
-        (prof, ant, r0, f0, grid, maxWr, img); returns the images."""
+        `scenes` is a sequence of (prof, ant, r0, f0, grid, maxWr[, img[, los]])
+        with img None or omitted to allocate it and los an optional
+        (los_x, los_y) tile hint; returns the images."""
         descs, keep = self._scene_descs(scenes)
         check(self.lib.sarf_bp_scenes(self._h, len(scenes), descs, tile, self._stream()))
         return keep
@@ -267,8 +268,9 @@ class Context:
         keep = []
         for i, sc in enumerate(scenes):
             prof, ant, r0, f0, grid, maxWr = sc[:6]
-            img = sc[6] if len(sc) > 6 else torch.empty((grid.Nx, grid.Ny), dtype=torch.complex64,
-                                                        device=prof.device)
+            img = sc[6] if len(sc) > 6 and sc[6] is not None else torch.empty(
+                (grid.Nx, grid.Ny), dtype=torch.complex64, device=prof.device)
+            los = sc[7] if len(sc) > 7 and sc[7] is not None else (0.0, 0.0)
             Np, nfft = prof.shape
             _need(prof, torch.complex64, (Np, nfft), f"scene {i} prof")
             _need(ant, torch.float64, (Np, 3), f"scene {i} ant")
@@ -277,7 +279,7 @@ class Context:
             _need(img, torch.complex64, (grid.Nx, grid.Ny), f"scene {i} img")
             descs[i] = _lib.SceneDesc(prof.data_ptr(), ant.data_ptr(), r0.data_ptr(),
                                       f0.data_ptr(), img.data_ptr(), grid.Nx, grid.Ny, nfft, Np,
-                                      grid.x0, grid.y0, grid.Wx, grid.Wy, maxWr, 0.0, 0.0)
+                                      grid.x0, grid.y0, grid.Wx, grid.Wy, maxWr, *los)
             keep.append(img)
         return descs, keep
 

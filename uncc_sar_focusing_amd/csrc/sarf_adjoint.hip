@@ -247,14 +247,22 @@ extern "C" int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *d, const 
         return fail(SARF_ERR_INVALID, "sarf_bp_entropy_grad: null argument");
     if (d->precision != SARF_BP_MIXED)
         return fail(SARF_ERR_UNSUPPORTED, "sarf_bp_entropy_grad: mixed precision only");
-    // default: the forward kernel's choice (tiles 6 and 7, 16 pixels per
-    // lane, are not instantiated here and fall back to 4)
+    // default: the forward kernel's choice mapped onto the tiles built here
+    // (lanes along y, at most 8 pixels per lane): 6, 7 -> 4; a lanes-along-x
+    // tile -> the lanes-along-y tile of the same footprint (11 -> 1,
+    // 12 -> 5, 13 -> 4, 14 -> 3)
     int tile = d->tile;
     if (tile <= 0) {
         BpArgs tmp{};
         SARF_TRY(bp_fill_args(ctx, d, 0, &tmp));
         tile = bp_pick_tile(d, tmp.dx, tmp.dy);
-        if (tile == 6 || tile == 7) tile = 4;
+        switch (tile) {
+            case 6: case 7: case 13: tile = 4; break;
+            case 11: tile = 1; break;
+            case 12: tile = 5; break;
+            case 14: tile = 3; break;
+            default: break;
+        }
     }
     BpArgs a{};
     SARF_TRY(bp_fill_args(ctx, d, tile, &a));

@@ -31,23 +31,24 @@ int bp_fill_args(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, BpArgs *a);
 
 namespace {
 
-template <int TY, int PX>
+template <int TY, int PX, bool XL>
 __global__ __launch_bounds__(kBpThreads) void bp_batch_entropy(
     BpArgs a, const float2 *__restrict__ prof, const double *__restrict__ ant,
     const double *__restrict__ r0, const double *__restrict__ f0, const double *__restrict__ dir,
     const double *__restrict__ basis, const double *__restrict__ coef, int nbasis, int ncand,
     double2 *__restrict__ partial) {
     __shared__ BpLds L;
+    __shared__ __align__(16) float fx[BpTile<TY, PX, XL>::kFx];
     __shared__ double2 red[kBpThreads / 64];
     const int ntiles = a.tiles_x * a.tiles_y;
     const long nitems = (long)ntiles * ncand;
     for (long item = blockIdx.x; item < nitems; item += gridDim.x) {
         const int t = (int)(item / ncand);
         const int c = (int)(item - (long)t * ncand);
-        BpTile<TY, PX> T;
+        BpTile<TY, PX, XL> T;
         T.init(a, t);
         LosPolyPos pos{ant, dir, basis, coef + (size_t)c * nbasis, nbasis, a.Np};
-        T.run(a, L, prof, pos, r0, f0, 0, a.Np);
+        T.run(a, L, fx, prof, pos, r0, f0, 0, a.Np);
         double s = 0.0, tt = 0.0;
 #pragma unroll
         for (int j = 0; j < PX; j++) {
@@ -116,12 +117,12 @@ __global__ __launch_bounds__(256) void image_entropy_final(const double2 *__rest
     }
 }
 
-template <int TY, int PX>
+template <int TY, int PX, bool XL = false>
 void launch_batch_t(const BpArgs &a, int grid, const float2 *prof, const double *ant,
                     const double *r0, const double *f0, const double *dir, const double *basis,
                     const double *coef, int nbasis, int ncand, double2 *partial, hipStream_t s) {
-    hipLaunchKernelGGL((bp_batch_entropy<TY, PX>), dim3(grid), dim3(kBpThreads), 0, s, a, prof,
-                       ant, r0, f0, dir, basis, coef, nbasis, ncand, partial);
+    hipLaunchKernelGGL((bp_batch_entropy<TY, PX, XL>), dim3(grid), dim3(kBpThreads), 0, s, a,
+                       prof, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial);
 }
 
 }  // namespace
@@ -163,6 +164,10 @@ extern "C" int sarf_bp_batch_entropy(sarf_ctx *ctx, const sarf_bp_desc *d, const
         case 3: launch_batch_t<64, 8>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
         case 4: launch_batch_t<32, 8>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
         case 5: launch_batch_t<16, 4>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
+        case 11: launch_batch_t<32, 4, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
+        case 12: launch_batch_t<64, 4, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
+        case 13: launch_batch_t<64, 8, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
+        case 14: launch_batch_t<32, 8, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
         default:
             return fail(SARF_ERR_UNSUPPORTED, "sarf_bp_batch_entropy: tile %d not compiled for batches", tid);
     }

@@ -15,13 +15,14 @@
 namespace sarf {
 namespace {
 
-template <int TY, int PX>
+template <int TY, int PX, bool XL>
 __global__ __launch_bounds__(kBpThreads) void bp_mixed(BpArgs a, const float2 *__restrict__ prof,
                                                        const double *__restrict__ ant,
                                                        const double *__restrict__ r0,
                                                        const double *__restrict__ f0,
                                                        float2 *__restrict__ img) {
     __shared__ BpLds L;
+    __shared__ __align__(16) float fx[BpTile<TY, PX, XL>::kFx];
     // blockIdx.x = split * ntiles + tile: every split of a tile covers a
     // contiguous pulse range and writes its own partial image
     const int ntiles = a.tiles_x * a.tiles_y;
@@ -32,9 +33,9 @@ __global__ __launch_bounds__(kBpThreads) void bp_mixed(BpArgs a, const float2 *_
     const int nend = min(a.Np, nbeg + per_split);
     img += (size_t)split * a.split_stride;
 
-    BpTile<TY, PX> T;
+    BpTile<TY, PX, XL> T;
     T.init(a, t);
-    T.run(a, L, prof, PlainPos{ant}, r0, f0, nbeg, nend);
+    T.run(a, L, fx, prof, PlainPos{ant}, r0, f0, nbeg, nend);
 #pragma unroll
     for (int j = 0; j < PX; j++) {
         int x, y;
@@ -109,44 +110,60 @@ __global__ __launch_bounds__(256) void bp_f64(BpArgs a, const double2 *__restric
     }
 }
 
-// Tile shapes compiled in: (TY, PX) -> TX = 256/TY * PX.
+// Tile shapes compiled in: id, extents tx x ty (pixels), lanes TL along y
+// (xl = 0) or along x (xl = 1), PX pixels per lane along the other axis.
 struct TileShape {
-    int id, ty, px, tx;
+    int id, tx, ty, tl, px, xl;
 };
 constexpr TileShape kTiles[] = {
-    {1, 32, 4, 32},  // 32 x 32
-    {2, 64, 4, 16},  // 16 x 64
-    {3, 64, 8, 32},  // 32 x 64
-    {4, 32, 8, 64},  // 64 x 32
-    {5, 16, 4, 64},  // 64 x 16
-    {6, 64, 16, 64}, // 64 x 64
-    {7, 32, 16, 128},// 128 x 32
+    {1, 32, 32, 32, 4, 0},
+    {2, 16, 64, 64, 4, 0},
+    {3, 32, 64, 64, 8, 0},
+    {4, 64, 32, 32, 8, 0},
+    {5, 64, 16, 16, 4, 0},
+    {6, 64, 64, 64, 16, 0},
+    {7, 128, 32, 32, 16, 0},
+    {11, 32, 32, 32, 4, 1},
+    {12, 64, 16, 64, 4, 1},
+    {13, 64, 32, 64, 8, 1},
+    {14, 32, 64, 32, 8, 1},
 };
 
-template <int TY, int PX>
+template <int TY, int PX, bool XL = false>
 void launch_mixed_t(const BpArgs &a, int grid, const float2 *prof, const double *ant,
                     const double *r0, const double *f0, float2 *img, hipStream_t s) {
-    hipLaunchKernelGGL((bp_mixed<TY, PX>), dim3(grid), dim3(kBpThreads), 0, s, a, prof, ant, r0,
-                       f0, img);
+    hipLaunchKernelGGL((bp_mixed<TY, PX, XL>), dim3(grid), dim3(kBpThreads), 0, s, a, prof, ant,
+                       r0, f0, img);
 }
 
 }  // namespace
 
-int bp_tile_count() { return (int)(sizeof(kTiles) / sizeof(kTiles[0])); }
+std::vector<int> bp_tile_ids() {
+    std::vector<int> ids;
+    for (const TileShape &t : kTiles) ids.push_back(t.id);
+    return ids;
+}
 
 // Default tile: the first tile of a measured speed ranking (MI355X, 2048^2
 // at 0.05 m and 256^2 at 0.1 m) whose line-of-sight-projected half extent
 // |los_x| hx + |los_y| hy -- the largest |dR - dR_c| over the tile, which
 // sets the FP32 rounding of the local range -- stays within 2.6 m (about
 // 3e-5 relative L2 measured).  With no LOS hint each axis counts fully and
-// the bound is 3.2 m.  The tile's worst-case window must fit in LDS.
+// the bound is 3.2 m.  The tile's worst-case window must fit in LDS.  With a
+// hint, tiles whose lanes run along the cross-range axis rank first (their
+// window gathers are LDS broadcasts): lanes along x when |los_y| >= |los_x|.
 int bp_pick_tile(const sarf_bp_desc *d, double dx, double dy) {
-    static const int kRank[] = {4, 5, 1, 3, 7, 2, 6};
+    static const int kRankY[] = {4, 5, 1, 3, 7, 2, 6};
+    static const int kRankX[] = {13, 4, 5, 1, 3, 14, 12, 11, 7, 2, 6};
     const double invbin = d->Nfft / d->maxWr;
     const bool hint = (d->los_x != 0.0 || d->los_y != 0.0);
     const double lx = hint ? std::fabs(d->los_x) : 1.0, ly = hint ? std::fabs(d->los_y) : 1.0;
     const double bound = hint ? 2.6 : 3.2;
-    for (int id : kRank) {
+    const bool xlanes = hint && ly >= lx;
+    const int *rank = xlanes ? kRankX : kRankY;
+    const int nrank = xlanes ? (int)(sizeof(kRankX) / sizeof(int)) : (int)(sizeof(kRankY) / sizeof(int));
+    for (int k = 0; k < nrank; k++) {
+        const int id = rank[k];
         for (const TileShape &t : kTiles) {
             if (t.id != id) continue;
             const double hx = 0.5 * (t.tx - 1) * dx, hy = 0.5 * (t.ty - 1) * dy;
@@ -241,13 +258,15 @@ std::array<uint64_t, 8> tune_key(const sarf_bp_desc *d) {
             bits(d->los_x) * 31 + bits(d->los_y)};
 }
 
-// Default pulse splits: with fewer than 4 tiles per CU, split the pulses
-// into groups that form partial images (summed in order afterwards),
-// keeping at least 64 pulses per group.
+// Default pulse splits: split the pulses into groups that form partial
+// images (summed in order afterwards) until there are 64 workgroups per CU,
+// keeping at least 16 pulses per group.  Measured on MI355X (tile 13): 2048^2
+// x 4096 pulses best at 8 groups (7.54 ms vs 8.18 unsplit), 512^2 x 1024 at
+// 16-32, 256^2 x 512 at 16.
 int default_splits(const sarf_ctx *ctx, int ntiles, int Np) {
     int splits = 1;
-    const int want = 4 * (ctx->cu_count > 0 ? ctx->cu_count : 256);
-    while (splits < 32 && ntiles * splits < want && Np / (2 * splits) >= 64) splits *= 2;
+    const long want = 64L * (ctx->cu_count > 0 ? ctx->cu_count : 256);
+    while (splits < 32 && (long)ntiles * splits < want && Np / (2 * splits) >= 16) splits *= 2;
     return splits;
 }
 
@@ -286,6 +305,10 @@ int launch_mixed_shape(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, int split
         case 5: launch_mixed_t<16, 4>(a, grid, p, ant, r0, f0, o, s); break;
         case 6: launch_mixed_t<64, 16>(a, grid, p, ant, r0, f0, o, s); break;
         case 7: launch_mixed_t<32, 16>(a, grid, p, ant, r0, f0, o, s); break;
+        case 11: launch_mixed_t<32, 4, true>(a, grid, p, ant, r0, f0, o, s); break;
+        case 12: launch_mixed_t<64, 4, true>(a, grid, p, ant, r0, f0, o, s); break;
+        case 13: launch_mixed_t<64, 8, true>(a, grid, p, ant, r0, f0, o, s); break;
+        case 14: launch_mixed_t<32, 8, true>(a, grid, p, ant, r0, f0, o, s); break;
         default: return fail(SARF_ERR_INVALID, "backproject: unknown tile id %d", tid);
     }
     SARF_HIP(hipGetLastError());
@@ -349,7 +372,7 @@ namespace {
 // Tiles that meet the default choice's accuracy bound (and the LDS window).
 std::vector<int> tune_tiles(const sarf_bp_desc *d) {
     std::vector<int> out;
-    for (int id = 1; id <= bp_tile_count(); id++)
+    for (int id : bp_tile_ids())
         if (bp_tile_accurate(d, id)) out.push_back(id);
     return out;
 }

@@ -25,6 +25,16 @@
 //         dR - dR_c = -s + t (g0 + g1 s),  g0 = 1/(2 R_c), g1 = 1/(2 R_c^2)
 //     (expansion of (R_c - s) sqrt(1 + t/(R_c - s)^2) - R_c; remainder
 //     t^2/(8 R^3)), so no square root per pixel.
+//   * Lane orientation: lanes run along y (XL = false) or along x (XL =
+//     true).  Lanes along the cross-range axis read nearly the same profile
+//     bin, so the per-pixel window gathers are LDS broadcasts; lanes along
+//     the line of sight spread over ~1.4 bins per lane and bank-conflict.
+//   * XL tiles use the same expansion in separable form: with P the pixel-
+//     axis and L the lane-axis coordinate (uh components u_p, u_l),
+//         dR - dR_c = F_p(P) + F_l(L) + P C1(L) + P^2 C2(L),
+//     F_p staged per (pulse, column) in LDS and F_l folded into the lane's
+//     window-position and phase offsets, so a pixel needs two FMAs for its
+//     local range instead of four.
 //   * Profile windows are staged in LDS as intercept and slope in
 //     window-local bin units, {s_i - i d_i, d_i} with d_i = s_{i+1} - s_i,
 //     so the lerp at position p (i = floor p) is one 16-byte LDS read and
@@ -64,6 +74,8 @@ struct BpLds {
     float4 pA[kBpChunk];  // uh_x, uh_y, g0, g1
     float4 pB[kBpChunk];  // pos_c (window-local), psi_c, kappa, W - 2
     int4 pI[kBpChunk];    // start bin, LDS offset (bins), W, interior
+    float4 pC[kBpChunk];  // separable terms (lane axis): c2l, c3l, e1, e2
+    float4 pD[kBpChunk];  // separable terms: e3, c2p, c3p (pixel axis), -
     int nc;
 };
 
@@ -99,48 +111,65 @@ __device__ __forceinline__ int swizzle_tile(int b, int ntiles) {
     return (b & 7) * per + (b >> 3);
 }
 
-template <int TY, int PX>
+// XL = false: the TY lanes of a group run along y and each lane owns PX
+// pixels along x (tile (256/TY * PX) x TY).  XL = true: the TY lanes run
+// along x and each lane owns PX pixels along y (tile TY x (256/TY * PX)).
+template <int TY, int PX, bool XL = false>
 struct BpTile {
-    static constexpr int GX = kBpThreads / TY;  // lane groups along x
-    static constexpr int TX = GX * PX;
+    static constexpr int TL = TY;                  // lanes along the lane axis
+    static constexpr int GX = kBpThreads / TL;     // lane groups along the pixel axis
+    static constexpr int NP = GX * PX;             // tile extent along the pixel axis
+    static constexpr int TX = XL ? TL : NP;        // tile extent along x
+    static constexpr int TYE = XL ? NP : TL;       // tile extent along y
+    static constexpr bool kSep = XL;               // separable local range
+    static constexpr int kFx = kSep ? kBpChunk * NP : 4;  // LDS floats for F_p
     int ix0, iy0, ly, gx;
     double pcx, pcy, hx, hy;
-    float qy;
-    float qx[PX], q2[PX], accr[PX], acci[PX];
+    float qy;  // lane coordinate, tile-centred (y if !XL, x if XL)
+    float qx[PX], q2[PX], x2[PX], accr[PX], acci[PX];  // pixel-axis coordinates
 
     __device__ __forceinline__ void init(const BpArgs &a, int t) {
         const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
         ix0 = tx * TX;
-        iy0 = ty * TY;
+        iy0 = ty * TYE;
         pcx = a.left + (ix0 + 0.5 * (TX - 1)) * a.dx;
-        pcy = a.bottom + (iy0 + 0.5 * (TY - 1)) * a.dy;
+        pcy = a.bottom + (iy0 + 0.5 * (TYE - 1)) * a.dy;
         hx = 0.5 * (TX - 1) * a.dx;
-        hy = 0.5 * (TY - 1) * a.dy;
+        hy = 0.5 * (TYE - 1) * a.dy;
         const int tid = threadIdx.x;
-        ly = tid % TY;
-        gx = tid / TY;
-        qy = (float)((ly - 0.5 * (TY - 1)) * a.dy);
+        ly = tid % TL;
+        gx = tid / TL;
+        const double dl = XL ? a.dx : a.dy, dp = XL ? a.dy : a.dx;
+        qy = (float)((ly - 0.5 * (TL - 1)) * dl);
 #pragma unroll
         for (int j = 0; j < PX; j++) {
-            qx[j] = (float)((gx * PX + j - 0.5 * (TX - 1)) * a.dx);
+            qx[j] = (float)((gx * PX + j - 0.5 * (NP - 1)) * dp);
             q2[j] = qx[j] * qx[j] + qy * qy;
+            x2[j] = qx[j] * qx[j];
             accr[j] = 0.f;
             acci[j] = 0.f;
         }
     }
 
     __device__ __forceinline__ bool pixel(int j, int &x, int &y, const BpArgs &a) const {
-        x = ix0 + gx * PX + j;
-        y = iy0 + ly;
+        if (XL) {
+            x = ix0 + ly;
+            y = iy0 + gx * PX + j;
+        } else {
+            x = ix0 + gx * PX + j;
+            y = iy0 + ly;
+        }
         return x < a.Nx && y < a.Ny;
     }
 
     // Per-(tile,pulse) FP64 set-up for pulses [n0, n0 + cmax): fills L.pA,
-    // L.pB, L.pI, packs windows, sets L.nc; then stages the windows.
+    // L.pB, L.pI, L.pC, L.pD, packs windows, sets L.nc; then stages the
+    // windows and, for separable tiles given fx, F_p per (pulse, column).
     template <class Pos>
     __device__ __forceinline__ void stage(const BpArgs &a, BpLds &L, const float2 *__restrict__ prof,
                                           const Pos &pos, const double *__restrict__ r0,
-                                          const double *__restrict__ f0, int n0, int cmax) {
+                                          const double *__restrict__ f0, int n0, int cmax,
+                                          float *fx = nullptr) {
         const int tid = threadIdx.x;
         __syncthreads();  // previous users of L are done
         if (tid < cmax) {
@@ -165,6 +194,22 @@ struct BpTile {
                                     (float)(0.5 * iR * iR));
             L.pB[tid] = make_float4((float)(posg - (double)start), (float)psi, (float)kap,
                                     (float)(W - 2));
+            {
+                // separable form of the same third-order expansion (u_p, u_l:
+                // uh along the pixel and lane axes):
+                //   F_p = -u_p P + g0 (1-u_p^2) P^2 + g1 u_p (1-u_p^2) P^3  (c2p, c3p)
+                //   F_l = -u_l L + g0 (1-u_l^2) L^2 + g1 u_l (1-u_l^2) L^3  (c2l, c3l)
+                //   C1  = -2 u_p u_l g0 L + g1 u_p (1-3u_l^2) L^2            (e1, e2)
+                //   C2  = g1 u_l (1-3u_p^2) L                                (e3)
+                const double ah = ux * iR, bh = uy * iR, g0 = 0.5 * iR, g1 = 0.5 * iR * iR;
+                const double up = XL ? bh : ah, ul = XL ? ah : bh;
+                const double mp = 1.0 - up * up, ml = 1.0 - ul * ul;
+                L.pC[tid] = make_float4((float)(g0 * ml), (float)(g1 * ul * ml),
+                                        (float)(-2.0 * up * ul * g0),
+                                        (float)(g1 * up * (1.0 - 3.0 * ul * ul)));
+                L.pD[tid] = make_float4((float)(g1 * ul * (1.0 - 3.0 * up * up)), (float)(g0 * mp),
+                                        (float)(g1 * up * mp), 0.f);
+            }
             // Fast path: the window lies inside the profile (no swath gate) and
             // |psi| <= 0.5 + kap * ext stays inside v_sin/v_cos's input range
             // (|x| <= 256 revolutions), so the phase needs no reduction.
@@ -198,18 +243,105 @@ struct BpTile {
                 L.win[I.y + b] = make_float4(fmaf(-fb, dr, s0.x), fmaf(-fb, di, s0.y), dr, di);
             }
         }
+        if (kSep && fx != nullptr) {
+            static_assert(kBpThreads % NP == 0, "a thread keeps one column");
+            const int col = tid % NP;
+            const double dp = XL ? a.dy : a.dx;
+            const float P = (float)((col - 0.5 * (NP - 1)) * dp);  // == qx of that column
+            for (int c = tid / NP; c < nc; c += kBpThreads / NP) {
+                const float4 A = L.pA[c];
+                const float up = XL ? A.y : A.x;
+                const float4 D = L.pD[c];
+                fx[c * NP + col] = P * fmaf(P, fmaf(P, D.z, D.y), -up);
+            }
+        }
         __syncthreads();
     }
 
-    // Accumulate the staged chunk into the lane's pixels.
-    __device__ __forceinline__ void accumulate_chunk(const BpArgs &a, const BpLds &L) {
+    // Separable form (XL tiles): per pulse the lane forms F_l, C1, C2 and
+    // folds F_l into its window-position and phase offsets; per pixel
+    // dl = F_p + P C1 + P^2 C2 = dR - dR_c - F_l.
+    __device__ __forceinline__ void accumulate_chunk_sep(const BpArgs &a, const BpLds &L,
+                                                         const float *__restrict__ fx) {
         const float invbin = a.invbin;
         const int nc = L.nc;
         for (int c = 0; c < nc; c++) {
             const float4 A = L.pA[c];
             const float4 B = L.pB[c];
             const int4 I = L.pI[c];
-            const float sy = A.y * qy;
+            const float4 C = L.pC[c];
+            const float e3 = L.pD[c].x;
+            const float ul = XL ? A.x : A.y;
+            const float Fl = qy * fmaf(qy, fmaf(qy, C.y, C.x), -ul);
+            const float C1 = qy * fmaf(qy, C.w, C.z);
+            const float C2 = qy * e3;
+            const float bx = fmaf(Fl, invbin, B.x);  // window position at dl = 0
+            const float by = fmaf(Fl, B.z, B.y);     // phase (revolutions) at dl = 0
+            float F[PX];
+#pragma unroll
+            for (int j = 0; j < PX; j += 4) {
+                const float4 f4 = *reinterpret_cast<const float4 *>(fx + c * NP + gx * PX + j);
+                F[j] = f4.x;
+                F[j + 1] = f4.y;
+                F[j + 2] = f4.z;
+                F[j + 3] = f4.w;
+            }
+            const float wmax = B.w;
+            if (I.w) {
+                // interior: no gate, no clamp (as in accumulate_chunk)
+#pragma unroll
+                for (int j = 0; j < PX; j++) {
+                    const float dl = fmaf(x2[j], C2, fmaf(qx[j], C1, F[j]));
+                    const float p = fmaf(dl, invbin, bx);
+                    const int i = (int)p;
+                    const float4 w = L.win[I.y + i];
+                    const float vr = fmaf(p, w.z, w.x);
+                    const float vi = fmaf(p, w.w, w.y);
+                    const float ph = fmaf(dl, B.z, by);  // revolutions
+                    const float sn = __builtin_amdgcn_sinf(ph);
+                    const float cs = __builtin_amdgcn_cosf(ph);
+                    accr[j] = fmaf(vr, cs, fmaf(-vi, sn, accr[j]));
+                    acci[j] = fmaf(vr, sn, fmaf(vi, cs, acci[j]));
+                }
+            } else {
+                // swath gate and one-revolution phase (as in accumulate_chunk)
+                const float glo = (float)(-I.x);
+                const float ghi = (float)(a.Nfft - 1 - I.x);
+#pragma unroll
+                for (int j = 0; j < PX; j++) {
+                    const float dl = fmaf(x2[j], C2, fmaf(qx[j], C1, F[j]));
+                    const float p = fmaf(dl, invbin, bx);
+                    const int i = (int)__builtin_amdgcn_fmed3f(p, 0.f, wmax);
+                    const float4 w = L.win[I.y + i];
+                    const bool ok = p > glo && p < ghi;
+                    const float vr = ok ? fmaf(p, w.z, w.x) : 0.f;
+                    const float vi = ok ? fmaf(p, w.w, w.y) : 0.f;
+                    const float ph = __builtin_amdgcn_fractf(fmaf(dl, B.z, by));
+                    const float sn = __builtin_amdgcn_sinf(ph);
+                    const float cs = __builtin_amdgcn_cosf(ph);
+                    accr[j] = fmaf(vr, cs, fmaf(-vi, sn, accr[j]));
+                    acci[j] = fmaf(vr, sn, fmaf(vi, cs, acci[j]));
+                }
+            }
+        }
+    }
+
+    // Accumulate the staged chunk into the lane's pixels.
+    __device__ __forceinline__ void accumulate_chunk(const BpArgs &a, const BpLds &L,
+                                                     const float *__restrict__ fx) {
+        if constexpr (kSep) {
+            accumulate_chunk_sep(a, L, fx);
+            return;
+        }
+        const float invbin = a.invbin;
+        const int nc = L.nc;
+        for (int c = 0; c < nc; c++) {
+            const float4 A = L.pA[c];
+            const float4 B = L.pB[c];
+            const int4 I = L.pI[c];
+            const float ul = XL ? A.x : A.y;  // uh along the lane axis
+            const float up = XL ? A.y : A.x;  // uh along the pixel axis
+            const float sy = ul * qy;
             const float wmax = B.w;
             if (I.w) {
                 // Interior: the window has one bin of margin on each side of
@@ -218,7 +350,7 @@ struct BpTile {
                 // truncating conversion needs no clamp.
 #pragma unroll
                 for (int j = 0; j < PX; j++) {
-                    const float s = fmaf(A.x, qx[j], sy);
+                    const float s = fmaf(up, qx[j], sy);
                     const float tq = fmaf(-s, s, q2[j]);
                     const float g = fmaf(s, A.w, A.z);
                     const float dl = fmaf(tq, g, -s);  // dR - dR_c
@@ -241,7 +373,7 @@ struct BpTile {
                 const float ghi = (float)(a.Nfft - 1 - I.x);
 #pragma unroll
                 for (int j = 0; j < PX; j++) {
-                    const float s = fmaf(A.x, qx[j], sy);
+                    const float s = fmaf(up, qx[j], sy);
                     const float tq = fmaf(-s, s, q2[j]);
                     const float g = fmaf(s, A.w, A.z);
                     const float dl = fmaf(tq, g, -s);
@@ -261,14 +393,15 @@ struct BpTile {
         }
     }
 
-    // All pulses in [nbeg, nend).
+    // All pulses in [nbeg, nend); fx: LDS of kFx floats.
     template <class Pos>
-    __device__ __forceinline__ void run(const BpArgs &a, BpLds &L, const float2 *__restrict__ prof,
-                                        const Pos &pos, const double *__restrict__ r0,
+    __device__ __forceinline__ void run(const BpArgs &a, BpLds &L, float *fx,
+                                        const float2 *__restrict__ prof, const Pos &pos,
+                                        const double *__restrict__ r0,
                                         const double *__restrict__ f0, int nbeg, int nend) {
         for (int n0 = nbeg; n0 < nend;) {
-            stage(a, L, prof, pos, r0, f0, n0, min(kBpChunk, nend - n0));
-            accumulate_chunk(a, L);
+            stage(a, L, prof, pos, r0, f0, n0, min(kBpChunk, nend - n0), fx);
+            accumulate_chunk(a, L, fx);
             n0 += L.nc;
         }
     }

@@ -36,12 +36,13 @@ __global__ void reset_counter(unsigned int *c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
 }
 
-template <int TY, int PX>
+template <int TY, int PX, bool XL>
 __global__ __launch_bounds__(kBpThreads) void bp_scenes(const SceneDev *__restrict__ scenes,
                                                         const int2 *__restrict__ items,
                                                         int nitems,
                                                         unsigned int *__restrict__ counter) {
     __shared__ BpLds L;
+    __shared__ __align__(16) float fx[BpTile<TY, PX, XL>::kFx];
     __shared__ int sItem;
     for (;;) {
         __syncthreads();  // previous item done with sItem and L
@@ -52,9 +53,9 @@ __global__ __launch_bounds__(kBpThreads) void bp_scenes(const SceneDev *__restri
         const int2 w = items[it];
         const SceneDev &S = scenes[w.x];
         const BpArgs a = S.a;
-        BpTile<TY, PX> T;
+        BpTile<TY, PX, XL> T;
         T.init(a, w.y);
-        T.run(a, L, S.prof, PlainPos{S.ant}, S.r0, S.f0, 0, a.Np);
+        T.run(a, L, fx, S.prof, PlainPos{S.ant}, S.r0, S.f0, 0, a.Np);
 #pragma unroll
         for (int j = 0; j < PX; j++) {
             int x, y;
@@ -63,12 +64,20 @@ __global__ __launch_bounds__(kBpThreads) void bp_scenes(const SceneDev *__restri
     }
 }
 
-template <int TY, int PX>
+template <int TY, int PX, bool XL = false>
 void launch_scenes_t(int grid, const SceneDev *sc, const int2 *items, int nitems,
                      unsigned int *counter, hipStream_t s) {
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, s, counter);
-    hipLaunchKernelGGL((bp_scenes<TY, PX>), dim3(grid), dim3(kBpThreads), 0, s, sc, items, nitems,
-                       counter);
+    hipLaunchKernelGGL((bp_scenes<TY, PX, XL>), dim3(grid), dim3(kBpThreads), 0, s, sc, items,
+                       nitems, counter);
+}
+
+// Tile ids compiled for batches (at most 8 pixels per lane).
+constexpr int kSceneTiles[] = {1, 2, 3, 4, 5, 11, 12, 13, 14};
+bool scene_tile_ok(int id) {
+    for (int t : kSceneTiles)
+        if (t == id) return true;
+    return false;
 }
 
 }  // namespace
@@ -119,7 +128,7 @@ extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scen
     // one tile shape for the whole batch: the given one, else the default
     int tid = tile > 0 ? tile : scenes_default_tile(nscenes, scenes);
     int tx = 0, ty = 0;
-    if (!bp_tile_shape(tid, &tx, &ty) || tid > 5)
+    if (!bp_tile_shape(tid, &tx, &ty) || !scene_tile_ok(tid))
         return fail(SARF_ERR_INVALID, "sarf_bp_scenes: tile %d not available for batches", tid);
     for (int i = 0; i < nscenes; i++) {
         const sarf_scene &c = scenes[i];
@@ -185,13 +194,17 @@ extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scen
         case 3: launch_scenes_t<64, 8>(grid, dsc, ditems, nit, counter, s); break;
         case 4: launch_scenes_t<32, 8>(grid, dsc, ditems, nit, counter, s); break;
         case 5: launch_scenes_t<16, 4>(grid, dsc, ditems, nit, counter, s); break;
+        case 11: launch_scenes_t<32, 4, true>(grid, dsc, ditems, nit, counter, s); break;
+        case 12: launch_scenes_t<64, 4, true>(grid, dsc, ditems, nit, counter, s); break;
+        case 13: launch_scenes_t<64, 8, true>(grid, dsc, ditems, nit, counter, s); break;
+        case 14: launch_scenes_t<32, 8, true>(grid, dsc, ditems, nit, counter, s); break;
     }
     SARF_HIP(hipGetLastError());
     return SARF_OK;
 }
 
-// Times sarf_bp_scenes with the default batch tile and every other tile
-// (1..5) that meets the accuracy bound for all scenes, and returns the
+// Times sarf_bp_scenes with the default batch tile and every other batch
+// tile that meets the accuracy bound for all scenes, and returns the
 // fastest (1 % hysteresis towards the default, then the smaller id); waits
 // for the stream.  The images hold the last run's result.
 extern "C" int sarf_autotune_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scenes, int reps,
@@ -209,7 +222,7 @@ extern "C" int sarf_autotune_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene
     }
     const int def = scenes_default_tile(nscenes, scenes);
     std::vector<int> cand{def};
-    for (int id = 1; id <= 5; id++) {
+    for (int id : kSceneTiles) {
         if (id == def) continue;
         bool ok = true;
         for (int i = 0; i < nscenes && ok; i++) {
