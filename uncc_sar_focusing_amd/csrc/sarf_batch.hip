@@ -32,7 +32,8 @@ int bp_fill_args(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, BpArgs *a);
 namespace {
 
 template <int TY, int PX, bool XL>
-__global__ __launch_bounds__(kBpThreads) void bp_batch_entropy(
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu<TY, PX, XL>())))
+void bp_batch_entropy(
     BpArgs a, const float2 *__restrict__ prof, const double *__restrict__ ant,
     const double *__restrict__ r0, const double *__restrict__ f0, const double *__restrict__ dir,
     const double *__restrict__ basis, const double *__restrict__ coef, int nbasis, int ncand,

@@ -16,7 +16,8 @@ namespace sarf {
 namespace {
 
 template <int TY, int PX, bool XL>
-__global__ __launch_bounds__(kBpThreads) void bp_mixed(BpArgs a, const float2 *__restrict__ prof,
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu<TY, PX, XL>())))
+void bp_mixed(BpArgs a, const float2 *__restrict__ prof,
                                                        const double *__restrict__ ant,
                                                        const double *__restrict__ r0,
                                                        const double *__restrict__ f0,

@@ -55,6 +55,14 @@
 namespace sarf {
 
 constexpr int kBpThreads = 256;
+// Waves per SIMD the tile kernels are compiled for: 4 (<= 128 VGPRs) up to
+// 8 pixels per lane, except lanes-along-x tiles whose column terms take the
+// LDS past 40 KiB (3 workgroups per CU); the 16-pixel tiles need more
+// registers than that.
+template <int TY, int PX, bool XL>
+constexpr int bp_waves_per_eu() {
+    return PX > 8 ? 1 : (XL && (kBpThreads / TY) * PX > 32) ? 3 : 4;
+}
 constexpr int kBpChunk = 32;  // max pulses staged per chunk
 constexpr int kBpCap = 2048;  // LDS window capacity, in bins (16 B each)
 

@@ -37,7 +37,8 @@ __global__ void reset_counter(unsigned int *c) {
 }
 
 template <int TY, int PX, bool XL>
-__global__ __launch_bounds__(kBpThreads) void bp_scenes(const SceneDev *__restrict__ scenes,
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu<TY, PX, XL>())))
+void bp_scenes(const SceneDev *__restrict__ scenes,
                                                         const int2 *__restrict__ items,
                                                         int nitems,
                                                         unsigned int *__restrict__ counter) {
