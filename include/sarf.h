@@ -114,7 +114,7 @@ typedef struct {
     int flags;      /* SARF_BP_ACCUMULATE */
     int tile;       /* 0 = default for the pixel spacing and line of sight; else a
                        tile id: 1..7 (lanes along y) or 11..14 (lanes along x);
-                       batches take 1..5 and 11..14, the gradient 1..5 */
+                       batches and the gradient take 1..5 and 11..14 */
     int pulse_split;/* 0 = automatic; n >= 1: pulses in n contiguous groups, each
                        forming a partial image, summed in group order */
     double los_x, los_y; /* optional hint: horizontal components of the unit line

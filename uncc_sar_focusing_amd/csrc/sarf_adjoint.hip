@@ -82,7 +82,7 @@ __device__ __forceinline__ int scatter_index(int lane) {
 
 constexpr int kWaves = kBpThreads / 64;
 
-template <int TY, int PX>
+template <int TY, int PX, bool XL>
 __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
     BpArgs a, const float2 *__restrict__ prof, const double *__restrict__ ant,
     const double *__restrict__ r0, const double *__restrict__ f0, const float2 *__restrict__ img,
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
     const int per_split = (a.Np + a.splits - 1) / a.splits;
     const int nbeg = min(a.Np, split * per_split);
     const int nend = min(a.Np, nbeg + per_split);
-    BpTile<TY, PX> T;
+    BpTile<TY, PX, XL> T;
     T.init(a, t);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
@@ -141,14 +141,16 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
                     const float4 A = L.pA[c];
                     const float4 B = L.pB[c];
                     const int4 I = L.pI[c];
-                    const float sy = A.y * T.qy;
+                    const float ul = XL ? A.x : A.y;  // uh along the lane axis
+                    const float up = XL ? A.y : A.x;  // uh along the pixel axis
+                    const float sy = ul * T.qy;
                     const float wmax = B.w;
                     const float kk = two_pi * B.z;  // rad per metre
                     const float glo = (float)(-I.x), ghi = (float)(a.Nfft - 1 - I.x);
                     float sA = 0.f, sB = 0.f, sC = 0.f;
 #pragma unroll
                     for (int j = 0; j < PX; j++) {
-                        const float s = fmaf(A.x, T.qx[j], sy);
+                        const float s = fmaf(up, T.qx[j], sy);
                         const float tq = fmaf(-s, s, T.q2[j]);
                         const float g = fmaf(s, A.w, A.z);
                         const float dl = fmaf(tq, g, -s);
@@ -172,10 +174,11 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
                         sB = fmaf(cpn, dl, sB);
                         sC = fmaf(cpn, T.qx[j], sC);
                     }
+                    // sC sums over the pixel axis, sA * qy is the lane axis
                     acc[4 * k + 0] = sA;
                     acc[4 * k + 1] = sB;
-                    acc[4 * k + 2] = sC;
-                    acc[4 * k + 3] = sA * T.qy;
+                    acc[4 * k + 2] = XL ? sA * T.qy : sC;  // Cx
+                    acc[4 * k + 3] = XL ? sC : sA * T.qy;  // Cy
                 }
             }
             const float r = wave_reduce_scatter(acc, lane);
@@ -224,11 +227,11 @@ __global__ __launch_bounds__(64) void adjoint_reduce(const double *__restrict__ 
     }
 }
 
-template <int TY, int PX>
+template <int TY, int PX, bool XL = false>
 void launch_adj_t(const BpArgs &a, int grid, const float2 *prof, const double *ant,
                   const double *r0, const double *f0, const float2 *img, const double *ent,
                   double *partial, hipStream_t s) {
-    hipLaunchKernelGGL((bp_entropy_adjoint<TY, PX>), dim3(grid), dim3(kBpThreads), 0, s, a, prof,
+    hipLaunchKernelGGL((bp_entropy_adjoint<TY, PX, XL>), dim3(grid), dim3(kBpThreads), 0, s, a, prof,
                        ant, r0, f0, img, ent, partial);
 }
 
@@ -247,22 +250,14 @@ extern "C" int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *d, const 
         return fail(SARF_ERR_INVALID, "sarf_bp_entropy_grad: null argument");
     if (d->precision != SARF_BP_MIXED)
         return fail(SARF_ERR_UNSUPPORTED, "sarf_bp_entropy_grad: mixed precision only");
-    // default: the forward kernel's choice mapped onto the tiles built here
-    // (lanes along y, at most 8 pixels per lane): 6, 7 -> 4; a lanes-along-x
-    // tile -> the lanes-along-y tile of the same footprint (11 -> 1,
-    // 12 -> 5, 13 -> 4, 14 -> 3)
+    // default: the forward kernel's choice (tiles 6 and 7, 16 pixels per
+    // lane, are not instantiated here and fall back to 4)
     int tile = d->tile;
     if (tile <= 0) {
         BpArgs tmp{};
         SARF_TRY(bp_fill_args(ctx, d, 0, &tmp));
         tile = bp_pick_tile(d, tmp.dx, tmp.dy);
-        switch (tile) {
-            case 6: case 7: case 13: tile = 4; break;
-            case 11: tile = 1; break;
-            case 12: tile = 5; break;
-            case 14: tile = 3; break;
-            default: break;
-        }
+        if (tile == 6 || tile == 7) tile = 4;
     }
     BpArgs a{};
     SARF_TRY(bp_fill_args(ctx, d, tile, &a));
@@ -292,6 +287,10 @@ extern "C" int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *d, const 
         case 3: launch_adj_t<64, 8>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
         case 4: launch_adj_t<32, 8>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
         case 5: launch_adj_t<16, 4>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
+        case 11: launch_adj_t<32, 4, true>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
+        case 12: launch_adj_t<64, 4, true>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
+        case 13: launch_adj_t<64, 8, true>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
+        case 14: launch_adj_t<32, 8, true>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
         default:
             return fail(SARF_ERR_UNSUPPORTED, "sarf_bp_entropy_grad: tile %d not compiled", tid);
     }
