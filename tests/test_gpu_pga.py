@@ -27,8 +27,10 @@ def _degraded_image(nx, ny, seed=4, amp=3.0):
     return img, bad.astype(np.complex64)
 
 
+# transform lengths M = 2^L >= 2 nx - 1: L = 7..9 and 5 take the in-LDS
+# row kernels, L = 10..13 the register-resident ones
 @pytest.mark.parametrize("nx,ny", [(64, 64), (75, 50), (128, 97), (333, 40), (1024, 31),
-                                   (4096, 6), (2, 9)])
+                                   (4096, 6), (2, 9), (600, 96), (700, 300), (1500, 16)])
 def test_pga_matches_oracle(ctx, nx, ny):
     img, bad = _degraded_image(nx, ny)
     iters = 3

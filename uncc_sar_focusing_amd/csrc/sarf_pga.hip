@@ -22,16 +22,20 @@
 // chirp-z identity inside one workgroup per row: nk = (n^2 + k^2 - (k-n)^2)/2
 // turns the DFT into a circular convolution with the chirp
 // w_m = exp(-j pi m^2 / N), evaluated with two power-of-two FFTs of length
-// M >= 2N - 1 on the row held in LDS (radix-2, in place); FFT(chirp) is
-// computed once per call.  The shift, mask and ifftshift of step 3 fold into
-// the load (index remap) and one phase factor per (k, y); step 6 is a
-// forward transform, the e^{-j phi} multiply and an inverse transform in the
-// same workgroup, in place.  Reductions over rows (column energy, phase
+// M >= 2N - 1 (M >= 32).  From M = 1024, M/16 lanes per row hold 16 values
+// each in registers and run self-sorting radix-16 passes through a padded LDS
+// row (two rows per workgroup at M = 1024, 2048); one transform's output
+// feeds the next in registers.  Shorter rows use one 256-thread workgroup per
+// row with the row in LDS (radix-2, in place), which has the lower latency
+// there.  FFT(chirp) is computed once per call (radix-2 in LDS).  The
+// shift, mask and ifftshift of step 3 fold into the load (index remap) and
+// one phase factor per (k, y); step 6 is a forward transform, the e^{-j phi}
+// multiply and an inverse transform in the same workgroup, in place.  Reductions over rows (column energy, phase
 // differences) are two-pass FP64 sums.
 #include <algorithm>
 #include <cmath>
 
-#include "sarf_internal.hpp"
+#include "sarf_fft.hpp"
 
 namespace sarf {
 namespace {
@@ -257,10 +261,13 @@ __global__ __launch_bounds__(1024) void pga_window(const double *__restrict__ w,
     for (int j = tid; j < Nx; j += blockDim.x) mask[j] = (j >= left && j <= right) ? 1.f : 0.f;
 }
 
-// Step 3 for row y (one workgroup, dynamic LDS of M float2):
+// Short rows (M <= 512): one workgroup of kThreads per row, the row in LDS
+// (dynamic, M float2), radix-2 transforms -- more lanes per row than the
+// register-resident kernels give, so lower latency at these lengths.
+// Step 3 for row y:
 // G[y][k] = e^{-j 2 pi k (s - fl)/N} DFT_x( mask[(x + s) mod N] At[y][x] )_k,
 // fl = floor(N/2): the shift by s, the mask and ifftshift as one remap.
-__global__ __launch_bounds__(kThreads) void pga_row_dft(const float2 *__restrict__ At, int N, int L,
+__global__ __launch_bounds__(kThreads) void pga_row_dft_lds(const float2 *__restrict__ At, int N, int L,
                                                         const int *__restrict__ shift,
                                                         const float *__restrict__ mask,
                                                         const float2 *__restrict__ ch2n,
@@ -299,7 +306,7 @@ __global__ __launch_bounds__(kThreads) void pga_row_dft(const float2 *__restrict
 
 // Step 6 for row y, in place: At[y] <- IDFT( DFT(At[y]) .* e^{-j phi} ),
 // the inverse as conj(DFT(conj(.)))/N through the same Bluestein core.
-__global__ __launch_bounds__(kThreads) void pga_row_apply(float2 *__restrict__ At, int N, int L,
+__global__ __launch_bounds__(kThreads) void pga_row_apply_lds(float2 *__restrict__ At, int N, int L,
                                                           const double2 *__restrict__ ephi,
                                                           const float2 *__restrict__ ch2n,
                                                           const float2 *__restrict__ twM,
@@ -333,6 +340,173 @@ __global__ __launch_bounds__(kThreads) void pga_row_apply(float2 *__restrict__ A
         const float2 u = cmul(buf[n], chirp(n, N, ch2n));
         row[n] = make_float2(u.x * scale, -u.y * scale);
     }
+}
+
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+// Bluestein's middle step on the forward transform of the chirped row held
+// in pass-0 order (v = conj of the input, as ifft_regs computes the forward
+// transform as conj(ifft(conj))): multiply by H and run the inverse
+// transform; on return v[r] = b_k (k = j + r*T) and X_k = w_k b_k / M.
+template <int LOG2M>
+__device__ __forceinline__ void bluestein_regs(float2 (&v)[16], float2 *Lr, int j,
+                                               const float2 *__restrict__ twM,
+                                               const float2 *__restrict__ H) {
+    constexpr int T = fft::Shape<LOG2M>::T;
+    fft::ifft_regs<LOG2M>(v, Lr, j, twM);  // v[r] = conj(A_k)
+    float2 u[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) u[r] = cmul(conjf2(v[r]), H[j + r * T]);
+    fft::to_input_order<LOG2M>(u, v);
+    fft::ifft_regs<LOG2M>(v, Lr, j, twM);
+}
+
+// Step 3 for rows y (P rows per workgroup, M/16 lanes each, M = 2^LOG2M):
+// G[y][k] = e^{-j 2 pi k (s - fl)/N} DFT_x( mask[(x + s) mod N] At[y][x] )_k,
+// fl = floor(N/2): the shift by s, the mask and ifftshift as one remap.
+template <int LOG2M>
+__global__ __launch_bounds__(fft::group_threads(LOG2M)) void pga_row_dft(
+    const float2 *__restrict__ At, int N, int Ny, const int *__restrict__ shift,
+    const float *__restrict__ mask, const float2 *__restrict__ ch2n,
+    const float2 *__restrict__ twM, const float2 *__restrict__ twN,
+    const float2 *__restrict__ H, float2 *__restrict__ G) {
+    using S = fft::Shape<LOG2M>;
+    constexpr int M = S::N, T = S::T, P = fft::rows_per_group(LOG2M), PAD = fft::padded(LOG2M);
+    __shared__ float2 lds[P * PAD];
+    const int lp = threadIdx.x / T, j = threadIdx.x % T;
+    const int y = blockIdx.x * P + lp;
+    const bool active = y < Ny;
+    float2 *Lr = lds + lp * PAD;
+    const int s = active ? shift[y] : 0;
+    const float2 *row = At + (size_t)(active ? y : 0) * N;
+    float2 v[16];
+#pragma unroll
+    for (int b = 0; b < S::NB0; b++)
+#pragma unroll
+        for (int r = 0; r < S::R0; r++) {
+            const int n = j + b * T + r * (M / S::R0);
+            float2 a = make_float2(0.f, 0.f);
+            if (active && n < N) {
+                int mi = n + s;
+                if (mi >= N) mi -= N;
+                const float m = mask[mi];
+                const float2 x = row[n];
+                a = cmul(make_float2(m * x.x, m * x.y), chirp(n, N, ch2n));
+            }
+            v[b * S::R0 + r] = conjf2(a);
+        }
+    bluestein_regs<LOG2M>(v, Lr, j, twM, H);
+    const float invM = 1.0f / (float)M;
+    const int fl = N / 2;
+    const long sh = ((long)(s - fl) % N + N) % N;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int k = j + r * T;
+        if (active && k < N) {
+            float2 X = cmul(v[r], chirp(k, N, ch2n));
+            X.x *= invM;
+            X.y *= invM;
+            const float2 e = twN[((long)k * sh) % N];  // e^{+j 2 pi k sh / N}; use its conjugate
+            G[(size_t)y * N + k] = cmul(X, conjf2(e));
+        }
+    }
+}
+
+// Step 6 for rows y, in place: At[y] <- IDFT( DFT(At[y]) .* e^{-j phi} ),
+// the inverse as conj(DFT(conj(.)))/N through the same Bluestein step.
+template <int LOG2M>
+__global__ __launch_bounds__(fft::group_threads(LOG2M)) void pga_row_apply(
+    float2 *__restrict__ At, int N, int Ny, const double2 *__restrict__ ephi,
+    const float2 *__restrict__ ch2n, const float2 *__restrict__ twM,
+    const float2 *__restrict__ H) {
+    using S = fft::Shape<LOG2M>;
+    constexpr int M = S::N, T = S::T, P = fft::rows_per_group(LOG2M), PAD = fft::padded(LOG2M);
+    __shared__ float2 lds[P * PAD];
+    const int lp = threadIdx.x / T, j = threadIdx.x % T;
+    const int y = blockIdx.x * P + lp;
+    const bool active = y < Ny;
+    float2 *Lr = lds + lp * PAD;
+    float2 *row = At + (size_t)(active ? y : 0) * N;
+    float2 v[16];
+#pragma unroll
+    for (int b = 0; b < S::NB0; b++)
+#pragma unroll
+        for (int r = 0; r < S::R0; r++) {
+            const int n = j + b * T + r * (M / S::R0);
+            const float2 a = (active && n < N) ? cmul(row[n], chirp(n, N, ch2n))
+                                                : make_float2(0.f, 0.f);
+            v[b * S::R0 + r] = conjf2(a);
+        }
+    bluestein_regs<LOG2M>(v, Lr, j, twM, H);
+    const float invM = 1.0f / (float)M;
+    // Z_k = X_k e^{-j phi_k}; the next forward transform's input is
+    // conj(Z_k) w_k, zero-padded, fed conjugated in pass-0 order
+    float2 u[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int k = j + r * T;
+        float2 nx = make_float2(0.f, 0.f);
+        if (k < N) {
+            const float2 w = chirp(k, N, ch2n);
+            const float2 X = cmul(v[r], w);
+            const double2 e = ephi[k];
+            const float2 Z = cmul(make_float2(X.x * invM, X.y * invM),
+                                  make_float2((float)e.x, (float)e.y));
+            nx = cmul(conjf2(Z), w);
+        }
+        u[r] = conjf2(nx);
+    }
+    fft::to_input_order<LOG2M>(u, v);
+    bluestein_regs<LOG2M>(v, Lr, j, twM, H);
+    const float scale = invM / (float)N;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int n = j + r * T;
+        if (active && n < N) {
+            const float2 uo = cmul(v[r], chirp(n, N, ch2n));
+            row[n] = make_float2(uo.x * scale, -uo.y * scale);
+        }
+    }
+}
+
+template <int LOG2M>
+void launch_rows(int Ny, bool apply, float2 *At, int N, const int *shift, const float *mask,
+                 const double2 *ephi, const float2 *ch2n, const float2 *twM, const float2 *twN,
+                 const float2 *H, float2 *G, hipStream_t s) {
+    constexpr int P = fft::rows_per_group(LOG2M);
+    const dim3 grid((Ny + P - 1) / P), block(fft::group_threads(LOG2M));
+    if (apply)
+        hipLaunchKernelGGL(pga_row_apply<LOG2M>, grid, block, 0, s, At, N, Ny, ephi, ch2n, twM, H);
+    else
+        hipLaunchKernelGGL(pga_row_dft<LOG2M>, grid, block, 0, s, At, N, Ny, shift, mask, ch2n,
+                           twM, twN, H, G);
+}
+
+// Row transforms (steps 3 and 6) for a transform length 2^L, 5 <= L <= 13:
+// one workgroup per row in LDS up to L = 9 (measured faster there), the
+// register-resident kernels from L = 10.
+int launch_row_kernel(int L, int Ny, bool apply, float2 *At, int N, const int *shift,
+                      const float *mask, const double2 *ephi, const float2 *ch2n,
+                      const float2 *twM, const float2 *twN, const float2 *H, float2 *G,
+                      hipStream_t s) {
+    if (L <= 9) {
+        const size_t lds = sizeof(float2) * ((size_t)1 << L);
+        if (apply)
+            hipLaunchKernelGGL(pga_row_apply_lds, dim3(Ny), dim3(kThreads), lds, s, At, N, L, ephi,
+                               ch2n, twM, H);
+        else
+            hipLaunchKernelGGL(pga_row_dft_lds, dim3(Ny), dim3(kThreads), lds, s, At, N, L, shift,
+                               mask, ch2n, twM, twN, H, G);
+        return SARF_OK;
+    }
+    switch (L) {
+#define SARF_PGA_ROWS(l) \
+        case l: launch_rows<l>(Ny, apply, At, N, shift, mask, ephi, ch2n, twM, twN, H, G, s); break;
+        SARF_PGA_ROWS(10) SARF_PGA_ROWS(11) SARF_PGA_ROWS(12) SARF_PGA_ROWS(13)
+#undef SARF_PGA_ROWS
+        default: return fail(SARF_ERR_UNSUPPORTED, "sarf_pga: transform length 2^%d", L);
+    }
+    return SARF_OK;
 }
 
 constexpr int kIntThreads = 1024;  // pga_integrate block; kPgaMaxN / 1024 = 4 per thread
@@ -447,7 +621,7 @@ extern "C" int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *d, void *img, double
         return fail(SARF_ERR_UNSUPPORTED, "sarf_pga: Nx=%d > %d", Nx, kPgaMaxN);
     hipStream_t s = (hipStream_t)stream;
     const int N = Nx;
-    int L = 1;
+    int L = 5;  // the row kernels take 32 <= M <= 8192
     while ((1 << L) < 2 * N - 1) L++;
     const int M = 1 << L;  // <= 8192
     const size_t npix = (size_t)Nx * Ny;
@@ -496,15 +670,15 @@ extern "C" int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *d, void *img, double
         hipLaunchKernelGGL(pga_col_final, dim3((N + kThreads - 1) / kThreads), dim3(kThreads), 0, s,
                            pa, pb, N, 0, w);
         hipLaunchKernelGGL(pga_window, dim3(1), dim3(1024), 0, s, w, N, d->db_down, mask);
-        hipLaunchKernelGGL(pga_row_dft, dim3(Ny), dim3(kThreads), lds, s, At, N, L, shift, mask,
-                           ch2n, twM, twN, H, G);
+        SARF_TRY(launch_row_kernel(L, Ny, false, At, N, shift, mask, ephi, ch2n, twM, twN, H, G,
+                                   s));
         hipLaunchKernelGGL(pga_col_partial, gcol, dim3(kThreads), 0, s, G, N, Ny, 1, pa, pb);
         hipLaunchKernelGGL(pga_col_final, dim3((N + kThreads - 1) / kThreads), dim3(kThreads), 0, s,
                            pa, pb, N, 1, phidot);
         hipLaunchKernelGGL(pga_integrate, dim3(1), dim3(kIntThreads), 0, s, phidot, N, ephi, af_ph,
                            rms + it);
-        hipLaunchKernelGGL(pga_row_apply, dim3(Ny), dim3(kThreads), lds, s, At, N, L, ephi, ch2n,
-                           twM, H);
+        SARF_TRY(launch_row_kernel(L, Ny, true, At, N, shift, mask, ephi, ch2n, twM, twN, H, G,
+                                   s));
         SARF_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(transpose_c64, dim3((Nx + 63) / 64, (Ny + 63) / 64), dim3(kThreads), 0, s,

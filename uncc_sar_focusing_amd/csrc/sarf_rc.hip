@@ -23,112 +23,14 @@
 #include <atomic>
 #include <cmath>
 
-#include "sarf_internal.hpp"
+#include "sarf_fft.hpp"
 
 namespace sarf {
 namespace {
 
-// cos(2 pi m / 16) for m = 0..15; sin(x) is cos(x - pi/2).
-template <int M>
-struct Tw16 {
-    static constexpr float c() {
-        constexpr float t[16] = {1.0f, 0.92387953251128674f, 0.70710678118654752f,
-                                 0.38268343236508978f, 0.0f, -0.38268343236508978f,
-                                 -0.70710678118654752f, -0.92387953251128674f, -1.0f,
-                                 -0.92387953251128674f, -0.70710678118654752f,
-                                 -0.38268343236508978f, 0.0f, 0.38268343236508978f,
-                                 0.70710678118654752f, 0.92387953251128674f};
-        return t[M & 15];
-    }
-    static constexpr float s() { return Tw16<(M + 12) & 15>::c(); }  // sin x = cos(x - pi/2)
-};
-
-// d * e^{SIGN j 2 pi M / 16}, with the trivial rotations done exactly.
-template <int SIGN, int M>
-__device__ __forceinline__ float2 rot16(float2 d) {
-    constexpr int m = M & 15;
-    if constexpr (m == 0) {
-        return d;
-    } else if constexpr (m == 8) {
-        return make_float2(-d.x, -d.y);
-    } else if constexpr (m == 4) {  // e^{SIGN j pi/2} = SIGN j
-        return SIGN > 0 ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
-    } else if constexpr (m == 12) {
-        return SIGN > 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
-    } else {
-        constexpr float c = Tw16<m>::c();
-        constexpr float s = SIGN * Tw16<m>::s();
-        return make_float2(d.x * c - d.y * s, d.x * s + d.y * c);
-    }
-}
-
-// Radix-2 decimation-in-frequency network on R registers (R = 2,4,8,16),
-// natural-order input, bit-reversed output.
-template <int SIGN, int L, int S, int K>
-struct DifBfly {
-    static __device__ __forceinline__ void run(float2 *v) {
-        if constexpr (K < L / 2) {
-            float2 a = v[S + K], b = v[S + K + L / 2];
-            v[S + K] = cadd(a, b);
-            v[S + K + L / 2] = rot16<SIGN, (K * 16 / L)>(csub(a, b));
-            DifBfly<SIGN, L, S, K + 1>::run(v);
-        }
-    }
-};
-template <int SIGN, int R, int L, int S>
-struct DifStage {
-    static __device__ __forceinline__ void run(float2 *v) {
-        if constexpr (S < R) {
-            DifBfly<SIGN, L, S, 0>::run(v);
-            DifStage<SIGN, R, L, S + L>::run(v);
-        }
-    }
-};
-template <int SIGN, int R, int L>
-struct DifAll {
-    static __device__ __forceinline__ void run(float2 *v) {
-        if constexpr (L >= 2) {
-            DifStage<SIGN, R, L, 0>::run(v);
-            DifAll<SIGN, R, L / 2>::run(v);
-        }
-    }
-};
-template <int R>
-__host__ __device__ constexpr int bitrev(int x) {
-    int r = 0;
-    for (int b = 1; b < R; b <<= 1) {
-        r = (r << 1) | (x & 1);
-        x >>= 1;
-    }
-    return r;
-}
-
-// In-place R-point DFT (sign SIGN) of v[0..R-1], result in natural order.
-template <int SIGN, int R>
-__device__ __forceinline__ void dft(float2 *v) {
-    DifAll<SIGN, R, R>::run(v);
-    float2 t[R];
-#pragma unroll
-    for (int i = 0; i < R; i++) t[i] = v[bitrev<R>(i)];
-#pragma unroll
-    for (int i = 0; i < R; i++) v[i] = t[i];
-}
-
-__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
-
-// w^r for r = 0..15 from w (r = 0 unused); error <= ~4 roundings.
-__device__ __forceinline__ void powers16(float2 w, float2 *p) {
-    p[1] = w;
-    p[2] = cmul(w, w);
-    p[3] = cmul(p[2], w);
-    p[4] = cmul(p[2], p[2]);
-    p[5] = cmul(p[4], w);
-    p[6] = cmul(p[4], p[2]);
-    p[7] = cmul(p[4], p[3]);
-    p[8] = cmul(p[4], p[4]);
-#pragma unroll
-    for (int r = 9; r < 16; r++) p[r] = cmul(p[8], p[r - 8]);
-}
+using fft::dft;
+using fft::lpad;
+using fft::powers16;
 
 constexpr int rc_threads(int log2n) {
     return ((1 << log2n) / 16 >= 256) ? (1 << log2n) / 16 : 256;
