@@ -291,6 +291,24 @@ int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *desc,
              double *rms,     /* device, iters */
              void *stream);
 
+/* PGA on many images of different sizes at once: every step of sarf_pga  */
+/* is one launch over all images (the row transforms one launch per       */
+/* transform length), with results identical to sarf_pga on each image.  */
+/* The images' device state is uploaded (and the call waits for the copy)  */
+/* only when the scene list differs from the previous call on this        */
+/* context, so after one call outside a stream capture the same call can  */
+/* be captured into a HIP graph.  No reference counterpart: the reference  */
+/* autofocuses one image per call (autofocus_cpu.cpp:249-474).            */
+typedef struct {
+    void *img;      /* device float2 Ny x Nx, updated in place */
+    double *af_ph;  /* device, Nx */
+    double *rms;    /* device, iters */
+    int Nx, Ny;     /* 2 <= Nx <= 4096, Ny >= 1 */
+} sarf_pga_scene;
+
+int sarf_pga_scenes(sarf_ctx *ctx, int nscenes, const sarf_pga_scene *scenes, int iters,
+                    double db_down, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Whole focusing call on host buffers: H2D, range compression,             */
 /* backprojection, D2H.  This is what cuda_focus_SAR_image<float,float>     */

@@ -19,7 +19,8 @@ def _dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
 
 
-def test_serving_batch_graph_replays_match_eager(ctx):
+@pytest.mark.parametrize("pga_mode", ["per_image", "batched"])
+def test_serving_batch_graph_replays_match_eager(ctx, pga_mode):
     specs = [(48, 96, ImageGrid(70, 45, 0.3, -0.2, 6.9, 4.4), 1024),
              (31, 64, ImageGrid(40, 52, -0.1, 0.2, 3.9, 5.1), 512)]
     scenes = []
@@ -33,12 +34,18 @@ def test_serving_batch_graph_replays_match_eager(ctx):
     descs = [(s["prof"], s["ant"], s["r0"], s["f0"], s["grid"], s["maxWr"], s["img"])
              for s in scenes]
 
+    outs = [(torch.empty(s["grid"].Nx, dtype=torch.float64, device=DEV),
+             torch.empty(2, dtype=torch.float64, device=DEV)) for s in scenes]
+
     def batch():
         for s in scenes:
             ctx.range_compress(s["ph"], s["nfft"], out=s["prof"])
         ctx.backproject_scenes(descs)
-        for s in scenes:
-            ctx.pga(s["img"], 2)
+        if pga_mode == "batched":
+            ctx.pga_scenes([s["img"] for s in scenes], 2, out=outs)
+        else:
+            for s in scenes:
+                ctx.pga(s["img"], 2)
 
     batch()  # eager: uploads the scene list, sizes every scratch buffer
     torch.cuda.synchronize()

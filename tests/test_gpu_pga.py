@@ -52,3 +52,29 @@ def test_pga_zero_iterations_is_identity(ctx):
     af_ph, rms = ctx.pga(t, 0)
     assert torch.equal(t.cpu(), torch.from_numpy(bad))
     assert float(af_ph.abs().max()) == 0.0
+
+
+def test_pga_scenes_equal_per_image_calls(ctx):
+    """sarf_pga_scenes on images of every transform length (M = 32 ... 8192,
+    both row-kernel families, uneven heights) gives exactly what sarf_pga
+    gives on each image alone; a second call with the same list reuses the
+    uploaded state and gives the same result again."""
+    sizes = [(64, 64), (75, 50), (333, 40), (1024, 31), (2, 9), (600, 96), (1500, 16), (4096, 6),
+             (130, 3)]
+    bads = [torch.from_numpy(_degraded_image(nx, ny, seed=7 + i)[1]).to(DEV)
+            for i, (nx, ny) in enumerate(sizes)]
+    iters = 3
+    single = []
+    for b in bads:
+        t = b.clone()
+        af_ph, rms = ctx.pga(t, iters)
+        single.append((t, af_ph.clone(), rms.clone()))
+    for rep in range(2):
+        imgs = [b.clone() for b in bads]
+        res = ctx.pga_scenes(imgs, iters)
+        torch.cuda.synchronize()
+        for (t, af_ph, rms), img, (a2, r2) in zip(single, imgs, res):
+            assert torch.equal(img, t), rep
+            assert torch.equal(a2, af_ph), rep
+            assert torch.equal(r2, rms), rep
+

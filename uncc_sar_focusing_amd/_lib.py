@@ -82,6 +82,7 @@ SIGNATURES = {
     "sarf_matched_filter": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sarf_image_db8": (_i, [_vp, _vp, _i, _i, C.c_float, _vp, _vp]),
     "sarf_autotune_scenes": (_i, [_vp, _i, C.c_void_p, _i, _vp, _vp, _vp]),
+    "sarf_pga_scenes": (_i, [_vp, _i, C.c_void_p, _i, _d, _vp]),
 }
 
 
@@ -109,6 +110,10 @@ class SceneDesc(C.Structure):
 
 class PgaDesc(C.Structure):
     _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("iters", C.c_int), ("db_down", C.c_double)]
+
+
+class PgaScene(C.Structure):
+    _fields_ = [("img", _vp), ("af_ph", _vp), ("rms", _vp), ("Nx", C.c_int), ("Ny", C.c_int)]
 
 
 def load(path: str = LIB_PATH) -> C.CDLL:

@@ -297,6 +297,30 @@ class Context:
                                 self._stream()))
         return af_ph, rms[:iters]
 
+    def pga_scenes(self, imgs, iters: int, db_down: float = 30.0, *, out=None):
+        """PGA on many images (each (Nx, Ny) complex64, sizes may differ) in
+        place, one launch per step for all of them (sarf_pga_scenes).  Returns
+        a list of (af_ph, rms) device tensors.  `out` may pass the list of
+        (af_ph, rms) tensors to write (for graph capture)."""
+        n = len(imgs)
+        descs = (_lib.PgaScene * n)()
+        res = []
+        for i, img in enumerate(imgs):
+            Nx, Ny = img.shape
+            _need(img, torch.complex64, (Nx, Ny), f"image {i}")
+            if out is not None:
+                af_ph, rms = out[i]
+                _need(af_ph, torch.float64, (Nx,), f"image {i} af_ph")
+                if rms.dtype != torch.float64 or rms.numel() < max(iters, 1):
+                    raise ValueError(f"image {i} rms: float64 with >= {max(iters, 1)} values")
+            else:
+                af_ph = torch.empty(Nx, dtype=torch.float64, device=img.device)
+                rms = torch.empty(max(iters, 1), dtype=torch.float64, device=img.device)
+            descs[i] = _lib.PgaScene(img.data_ptr(), af_ph.data_ptr(), rms.data_ptr(), Nx, Ny)
+            res.append((af_ph, rms))
+        check(self.lib.sarf_pga_scenes(self._h, n, descs, iters, db_down, self._stream()))
+        return [(a, r[:iters]) for a, r in res]
+
     # --------------------------------------------------------------- focus
     def focus_host(self, ph, ant, r0, f0, grid: ImageGrid, nfft: int, maxWr: float,
                    precision: str = "mixed"):

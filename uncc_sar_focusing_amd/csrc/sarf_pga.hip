@@ -1,4 +1,4 @@
-// sarf_pga.hip -- phase gradient autofocus (PGA) on a focused image.
+// sarf_pga.hip -- phase gradient autofocus (PGA) on focused images.
 //
 // Semantics of matlab/PGA.m:151-232 (the reference's C++ restatement is
 // src/autofocus/cpu/autofocus_cpu.cpp:249-474), image rows = range y,
@@ -19,21 +19,32 @@
 // GPU formulation.  The image is transposed once into rows along x
 // (At[y*Nx + x]) for all iterations and transposed back at the end.  Every
 // transform along x is a length-N DFT (any N <= 4096) done by Bluestein's
-// chirp-z identity inside one workgroup per row: nk = (n^2 + k^2 - (k-n)^2)/2
-// turns the DFT into a circular convolution with the chirp
-// w_m = exp(-j pi m^2 / N), evaluated with two power-of-two FFTs of length
-// M >= 2N - 1 (M >= 32).  From M = 1024, M/16 lanes per row hold 16 values
-// each in registers and run self-sorting radix-16 passes through a padded LDS
-// row (two rows per workgroup at M = 1024, 2048); one transform's output
-// feeds the next in registers.  Shorter rows use one 256-thread workgroup per
-// row with the row in LDS (radix-2, in place), which has the lower latency
-// there.  FFT(chirp) is computed once per call (radix-2 in LDS).  The
-// shift, mask and ifftshift of step 3 fold into the load (index remap) and
-// one phase factor per (k, y); step 6 is a forward transform, the e^{-j phi}
-// multiply and an inverse transform in the same workgroup, in place.  Reductions over rows (column energy, phase
-// differences) are two-pass FP64 sums.
+// chirp-z identity: nk = (n^2 + k^2 - (k-n)^2)/2 turns the DFT into a
+// circular convolution with the chirp w_m = exp(-j pi m^2 / N), evaluated
+// with two power-of-two FFTs of length M >= 2N - 1 (M >= 32).  From
+// M = 1024, M/16 lanes per row hold 16 values each in registers and run
+// self-sorting radix-16 passes through a padded LDS row (4 and 2 rows per
+// workgroup at M = 1024, 2048); one transform's output feeds the next in
+// registers.  Shorter rows use one 256-thread workgroup per row with the row
+// in LDS (radix-2, in place), which has the lower latency there.  FFT(chirp)
+// is computed once per call (radix-2 in LDS).  The shift, mask and
+// ifftshift of step 3 fold into the load (index remap) and one phase factor
+// per (k, y); step 6 is a forward transform, the e^{-j phi} multiply and an
+// inverse transform in the same workgroup, in place.  Reductions over rows
+// (column energy, phase differences) are two-pass FP64 sums.
+//
+// Every step is a device body on one image's state (PgaImage).  sarf_pga
+// launches the bodies for one image with the state as a kernel argument;
+// sarf_pga_scenes launches each step once for a whole batch of images of
+// different sizes: the state of every image sits in device memory, each
+// launch covers the concatenated blocks of all images (a block finds its
+// image by binary search over the per-image block offsets), and the row
+// transforms run one launch per transform length.  Both run the same
+// arithmetic in the same order, so their results are identical.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <vector>
 
 #include "sarf_fft.hpp"
 
@@ -42,7 +53,23 @@ namespace {
 
 constexpr int kPgaMaxN = 4096;
 constexpr int kThreads = 256;
-constexpr int kRowChunks = 16;  // y chunks of the two-pass column reductions
+constexpr int kRowChunks = 16;     // y chunks of the two-pass column reductions
+constexpr int kIntThreads = 1024;  // window / integrate block; kPgaMaxN / 1024 = 4 per thread
+constexpr int kMinL = 5, kMaxL = 13;
+
+// Device state of one image's PGA call.
+struct PgaImage {
+    float2 *img, *At, *G;  // image (Nx rows of Ny), its transpose, row spectra
+    double *af_ph, *rms;   // outputs: Nx, iters
+    int *shift;            // Ny
+    double *pa, *pb;       // kRowChunks x Nx partial column sums
+    double *w, *phidot;    // Nx
+    double2 *ephi;         // Nx
+    float *mask;           // Nx
+    float2 *H;             // M: FFT of the conjugate chirp
+    const float2 *twM, *twN, *ch2n;  // e^{+j 2 pi t/M}, /N, /(2N)
+    int Nx, Ny, L;
+};
 
 // In-place radix-2 decimation-in-time FFT of buf[0..M), M = 2^L, by one
 // workgroup of kThreads: forward uses e^{-j 2 pi t/M}, inverse e^{+j 2 pi t/M}
@@ -84,6 +111,8 @@ __device__ __forceinline__ float2 chirp(int m, int N, const float2 *__restrict__
     return make_float2(c.x, -c.y);
 }
 
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
 // buf[0..N) holds a_n = x_n w_n, buf[N..M) zeros; on return buf[k] (k < N)
 // holds X_k / w_k * M, i.e. X_k = w_k buf[k] / M is the DFT of x.
 __device__ __forceinline__ void bluestein_core(float2 *buf, int L, const float2 *__restrict__ twM,
@@ -95,29 +124,33 @@ __device__ __forceinline__ void bluestein_core(float2 *buf, int L, const float2 
     lds_fft(buf, L, twM, true);
 }
 
-// H = FFT_M(h), h_m = conj(w_m) for m < N, h_{M-m} = conj(w_m) for 0 < m < N.
-__global__ __launch_bounds__(kThreads) void pga_chirp_spectrum(int N, int L,
-                                                               const float2 *__restrict__ ch2n,
-                                                               const float2 *__restrict__ twM,
-                                                               float2 *__restrict__ H) {
+// ---------------------------------------------------------------- bodies
+
+// af_ph = 0 and, with `spectrum`, H = FFT_M(h), h_m = conj(w_m) for m < N,
+// h_{M-m} = conj(w_m) for 0 < m < N (one workgroup of kThreads, dynamic LDS
+// of at least M float2).
+__device__ void setup_body(const PgaImage &p, bool spectrum) {
     extern __shared__ float2 buf[];
-    const int M = 1 << L;
+    for (int i = threadIdx.x; i < p.Nx; i += kThreads) p.af_ph[i] = 0.0;
+    if (!spectrum) return;
+    const int N = p.Nx, L = p.L, M = 1 << L;
     for (int i = threadIdx.x; i < M; i += kThreads) {
         float2 v = make_float2(0.f, 0.f);
-        if (i < N) v = chirp(i, N, ch2n);
-        else if (i > M - N) v = chirp(M - i, N, ch2n);
-        buf[i] = make_float2(v.x, -v.y);
+        if (i < N) v = chirp(i, N, p.ch2n);
+        else if (i > M - N) v = chirp(M - i, N, p.ch2n);
+        buf[i] = conjf2(v);
     }
     __syncthreads();
-    lds_fft(buf, L, twM, false);
-    for (int i = threadIdx.x; i < M; i += kThreads) H[i] = buf[i];
+    lds_fft(buf, L, p.twM, false);
+    for (int i = threadIdx.x; i < M; i += kThreads) p.H[i] = buf[i];
 }
 
-// dst[c*R + r] = src[r*C + c] for an R x C complex matrix (64 x 64 tiles).
-__global__ __launch_bounds__(kThreads) void transpose_c64(const float2 *__restrict__ src, int R,
-                                                          int C, float2 *__restrict__ dst) {
+// dst[c*R + r] = src[r*C + c] for an R x C complex matrix; 64 x 64 tile
+// (bx, by) = (column tile, row tile).
+__device__ void transpose_body(const float2 *__restrict__ src, int R, int C,
+                               float2 *__restrict__ dst, int bx, int by) {
     __shared__ float2 tile[64][65];
-    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    const int r0 = by * 64, c0 = bx * 64;
     const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
     for (int i = ly; i < 64; i += kThreads / 64) {
         const int r = r0 + i, c = c0 + lx;
@@ -130,14 +163,13 @@ __global__ __launch_bounds__(kThreads) void transpose_c64(const float2 *__restri
     }
 }
 
-// Per row y (one workgroup): first maximum of |At[y][x]|^2 over x ->
-// circular shift s_y = ceil(N/2) - idx (mod N).
-__global__ __launch_bounds__(kThreads) void pga_row_argmax(const float2 *__restrict__ At, int N,
-                                                           int *__restrict__ shift) {
+// Row y: first maximum of |At[y][x]|^2 over x -> circular shift
+// s_y = ceil(N/2) - idx (mod N).
+__device__ void argmax_body(const PgaImage &p, int y) {
     __shared__ float bm[kThreads];
     __shared__ int bi[kThreads];
-    const int y = blockIdx.x, tid = threadIdx.x;
-    const float2 *row = At + (size_t)y * N;
+    const int N = p.Nx, tid = threadIdx.x;
+    const float2 *row = p.At + (size_t)y * N;
     float best = -1.f;
     int idx = 0;
     for (int x = tid; x < N; x += kThreads) {
@@ -164,30 +196,29 @@ __global__ __launch_bounds__(kThreads) void pga_row_argmax(const float2 *__restr
     }
     if (tid == 0) {
         const int center = (N + 1) / 2;  // 0-based target column = 1-based ceil(N/2)
-        shift[y] = ((center - bi[0]) % N + N) % N;
+        p.shift[y] = ((center - bi[0]) % N + N) % N;
     }
 }
 
-// Two-pass column sums over rows.  Pass 1, grid (ceil(N/64), kRowChunks):
-// part[chunk*N + x] = sum over the chunk's rows of |At[y][x]|^2 (mode 0), or
+// Two-pass column sums over rows.  Pass 1, block (bx, chunk):
+// pa[chunk*N + x] = sum over the chunk's rows of |At[y][x]|^2 (mode 0), or
 // for G (mode 1) num/den parts of phi_dot: Im(conj G_k G_{k+1}), |G_k|^2.
-__global__ __launch_bounds__(kThreads) void pga_col_partial(const float2 *__restrict__ At, int N,
-                                                            int Ny, int mode,
-                                                            double *__restrict__ pa,
-                                                            double *__restrict__ pb) {
+__device__ void col_partial_body(const PgaImage &p, int mode, int bx, int chunk) {
     __shared__ double ra[4][64], rb[4][64];
+    const float2 *src = mode == 0 ? p.At : p.G;
+    const int N = p.Nx, Ny = p.Ny;
     const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
-    const int x = blockIdx.x * 64 + lx;
+    const int x = bx * 64 + lx;
     const int per = (Ny + kRowChunks - 1) / kRowChunks;
-    const int yb = blockIdx.y * per, ye = min(Ny, yb + per);
+    const int yb = chunk * per, ye = min(Ny, yb + per);
     double a = 0.0, b = 0.0;
     if (x < N) {
         for (int y = yb + ly; y < ye; y += 4) {
-            const float2 g = At[(size_t)y * N + x];
+            const float2 g = src[(size_t)y * N + x];
             if (mode == 0) {
                 a += (double)g.x * g.x + (double)g.y * g.y;
             } else if (x + 1 < N) {
-                const float2 g1 = At[(size_t)y * N + x + 1];
+                const float2 g1 = src[(size_t)y * N + x + 1];
                 a += (double)g.x * g1.y - (double)g.y * g1.x;
                 b += (double)g.x * g.x + (double)g.y * g.y;
             }
@@ -197,41 +228,40 @@ __global__ __launch_bounds__(kThreads) void pga_col_partial(const float2 *__rest
     rb[ly][lx] = b;
     __syncthreads();
     if (ly == 0 && x < N) {
-        pa[(size_t)blockIdx.y * N + x] = (ra[0][lx] + ra[1][lx]) + (ra[2][lx] + ra[3][lx]);
-        if (mode == 1) pb[(size_t)blockIdx.y * N + x] = (rb[0][lx] + rb[1][lx]) + (rb[2][lx] + rb[3][lx]);
+        p.pa[(size_t)chunk * N + x] = (ra[0][lx] + ra[1][lx]) + (ra[2][lx] + ra[3][lx]);
+        if (mode == 1) p.pb[(size_t)chunk * N + x] = (rb[0][lx] + rb[1][lx]) + (rb[2][lx] + rb[3][lx]);
     }
 }
 
-// Pass 2: out[x] = sum of the chunk parts (mode 0: energy), or
-// phi_dot[k] = num / den for k < N-1 (mode 1).
-__global__ __launch_bounds__(kThreads) void pga_col_final(const double *__restrict__ pa,
-                                                          const double *__restrict__ pb, int N,
-                                                          int mode, double *__restrict__ out) {
-    const int x = blockIdx.x * kThreads + threadIdx.x;
+// Pass 2, block bx: w[x] = sum of the chunk parts (mode 0: energy), or
+// phidot[k] = num / den for k < N-1 (mode 1).
+__device__ void col_final_body(const PgaImage &p, int mode, int bx) {
+    const int N = p.Nx;
+    const int x = bx * kThreads + threadIdx.x;
     if (x >= N) return;
     double a = 0.0, b = 0.0;
     for (int c = 0; c < kRowChunks; c++) {
-        a += pa[(size_t)c * N + x];
-        if (mode == 1) b += pb[(size_t)c * N + x];
+        a += p.pa[(size_t)c * N + x];
+        if (mode == 1) b += p.pb[(size_t)c * N + x];
     }
-    if (mode == 0) out[x] = a;
-    else if (x < N - 1) out[x] = b > 0.0 ? a / b : 0.0;
+    if (mode == 0) p.w[x] = a;
+    else if (x < N - 1) p.phidot[x] = b > 0.0 ? a / b : 0.0;
 }
 
-// One workgroup: window limits from the column energy (mask[j] = 1 on
-// [left, right], 0-based): left = 1 + the last column before the centre
-// below the cut-off, right = the first column from the centre on below it,
-// minus 1 (parallel max/min reductions).
-__global__ __launch_bounds__(1024) void pga_window(const double *__restrict__ w, int Nx,
-                                                   double db_down, float *__restrict__ mask) {
-    __shared__ double rmax[1024];
-    __shared__ int rl[1024], rr[1024];
-    const int tid = threadIdx.x;
+// One workgroup of kIntThreads: window limits from the column energy
+// (mask[j] = 1 on [left, right], 0-based): left = 1 + the last column before
+// the centre below the cut-off, right = the first column from the centre on
+// below it, minus 1 (parallel max/min reductions).
+__device__ void window_body(const PgaImage &p, double db_down) {
+    __shared__ double rmax[kIntThreads];
+    __shared__ int rl[kIntThreads], rr[kIntThreads];
+    const int Nx = p.Nx, tid = threadIdx.x;
+    const double *w = p.w;
     double m = 0.0;
-    for (int x = tid; x < Nx; x += blockDim.x) m = fmax(m, w[x]);
+    for (int x = tid; x < Nx; x += kIntThreads) m = fmax(m, w[x]);
     rmax[tid] = m;
     __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    for (int o = kIntThreads / 2; o > 0; o >>= 1) {
         if (tid < o) rmax[tid] = fmax(rmax[tid], rmax[tid + o]);
         __syncthreads();
     }
@@ -240,7 +270,7 @@ __global__ __launch_bounds__(1024) void pga_window(const double *__restrict__ w,
     const double cut = -fabs(db_down);
     const int center = (Nx + 1) / 2;  // 1-based ceil(Nx/2)
     int lastb = -1, firstb = Nx;      // last below in [0, center), first below in [center, Nx)
-    for (int j = tid; j < Nx; j += blockDim.x) {
+    for (int j = tid; j < Nx; j += kIntThreads) {
         if (10.0 * log10(w[j] / wmax) < cut) {
             if (j < center) lastb = max(lastb, j);
             else firstb = min(firstb, j);
@@ -249,7 +279,7 @@ __global__ __launch_bounds__(1024) void pga_window(const double *__restrict__ w,
     rl[tid] = lastb;
     rr[tid] = firstb;
     __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    for (int o = kIntThreads / 2; o > 0; o >>= 1) {
         if (tid < o) {
             rl[tid] = max(rl[tid], rl[tid + o]);
             rr[tid] = min(rr[tid], rr[tid + o]);
@@ -258,258 +288,8 @@ __global__ __launch_bounds__(1024) void pga_window(const double *__restrict__ w,
     }
     const int left = rl[0] + 1;                       // 0 when none is below
     const int right = rr[0] < Nx ? rr[0] - 1 : Nx - 1;
-    for (int j = tid; j < Nx; j += blockDim.x) mask[j] = (j >= left && j <= right) ? 1.f : 0.f;
+    for (int j = tid; j < Nx; j += kIntThreads) p.mask[j] = (j >= left && j <= right) ? 1.f : 0.f;
 }
-
-// Short rows (M <= 512): one workgroup of kThreads per row, the row in LDS
-// (dynamic, M float2), radix-2 transforms -- more lanes per row than the
-// register-resident kernels give, so lower latency at these lengths.
-// Step 3 for row y:
-// G[y][k] = e^{-j 2 pi k (s - fl)/N} DFT_x( mask[(x + s) mod N] At[y][x] )_k,
-// fl = floor(N/2): the shift by s, the mask and ifftshift as one remap.
-__global__ __launch_bounds__(kThreads) void pga_row_dft_lds(const float2 *__restrict__ At, int N, int L,
-                                                        const int *__restrict__ shift,
-                                                        const float *__restrict__ mask,
-                                                        const float2 *__restrict__ ch2n,
-                                                        const float2 *__restrict__ twM,
-                                                        const float2 *__restrict__ twN,
-                                                        const float2 *__restrict__ H,
-                                                        float2 *__restrict__ G) {
-    extern __shared__ float2 buf[];
-    const int y = blockIdx.x, M = 1 << L;
-    const int s = shift[y];
-    const float2 *row = At + (size_t)y * N;
-    for (int x = threadIdx.x; x < M; x += kThreads) {
-        float2 v = make_float2(0.f, 0.f);
-        if (x < N) {
-            int mi = x + s;
-            if (mi >= N) mi -= N;
-            const float m = mask[mi];
-            const float2 a = row[x];
-            v = cmul(make_float2(m * a.x, m * a.y), chirp(x, N, ch2n));
-        }
-        buf[x] = v;
-    }
-    __syncthreads();
-    bluestein_core(buf, L, twM, H);
-    const float invM = 1.0f / (float)M;
-    const int fl = N / 2;
-    const long sh = ((long)(s - fl) % N + N) % N;
-    for (int k = threadIdx.x; k < N; k += kThreads) {
-        float2 X = cmul(buf[k], chirp(k, N, ch2n));
-        X.x *= invM;
-        X.y *= invM;
-        const float2 e = twN[((long)k * sh) % N];  // e^{+j 2 pi k sh / N}; use its conjugate
-        G[(size_t)y * N + k] = cmul(X, make_float2(e.x, -e.y));
-    }
-}
-
-// Step 6 for row y, in place: At[y] <- IDFT( DFT(At[y]) .* e^{-j phi} ),
-// the inverse as conj(DFT(conj(.)))/N through the same Bluestein core.
-__global__ __launch_bounds__(kThreads) void pga_row_apply_lds(float2 *__restrict__ At, int N, int L,
-                                                          const double2 *__restrict__ ephi,
-                                                          const float2 *__restrict__ ch2n,
-                                                          const float2 *__restrict__ twM,
-                                                          const float2 *__restrict__ H) {
-    extern __shared__ float2 buf[];
-    const int y = blockIdx.x, M = 1 << L;
-    float2 *row = At + (size_t)y * N;
-    for (int x = threadIdx.x; x < M; x += kThreads)
-        buf[x] = x < N ? cmul(row[x], chirp(x, N, ch2n)) : make_float2(0.f, 0.f);
-    __syncthreads();
-    bluestein_core(buf, L, twM, H);
-    const float invM = 1.0f / (float)M;
-    // Z_k = X_k e^{-j phi_k}; next input conj(Z_k) w_k, zero-padded (each
-    // lane reads and rewrites only its own elements)
-    for (int k = threadIdx.x; k < M; k += kThreads) {
-        float2 v = make_float2(0.f, 0.f);
-        if (k < N) {
-            const float2 w = chirp(k, N, ch2n);
-            const float2 X = cmul(buf[k], w);
-            const double2 e = ephi[k];
-            const float2 Z = cmul(make_float2(X.x * invM, X.y * invM),
-                                  make_float2((float)e.x, (float)e.y));
-            v = cmul(make_float2(Z.x, -Z.y), w);
-        }
-        buf[k] = v;
-    }
-    __syncthreads();
-    bluestein_core(buf, L, twM, H);
-    const float scale = invM / (float)N;
-    for (int n = threadIdx.x; n < N; n += kThreads) {
-        const float2 u = cmul(buf[n], chirp(n, N, ch2n));
-        row[n] = make_float2(u.x * scale, -u.y * scale);
-    }
-}
-
-__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
-
-// Bluestein's middle step on the forward transform of the chirped row held
-// in pass-0 order (v = conj of the input, as ifft_regs computes the forward
-// transform as conj(ifft(conj))): multiply by H and run the inverse
-// transform; on return v[r] = b_k (k = j + r*T) and X_k = w_k b_k / M.
-template <int LOG2M>
-__device__ __forceinline__ void bluestein_regs(float2 (&v)[16], float2 *Lr, int j,
-                                               const float2 *__restrict__ twM,
-                                               const float2 *__restrict__ H) {
-    constexpr int T = fft::Shape<LOG2M>::T;
-    fft::ifft_regs<LOG2M>(v, Lr, j, twM);  // v[r] = conj(A_k)
-    float2 u[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) u[r] = cmul(conjf2(v[r]), H[j + r * T]);
-    fft::to_input_order<LOG2M>(u, v);
-    fft::ifft_regs<LOG2M>(v, Lr, j, twM);
-}
-
-// Step 3 for rows y (P rows per workgroup, M/16 lanes each, M = 2^LOG2M):
-// G[y][k] = e^{-j 2 pi k (s - fl)/N} DFT_x( mask[(x + s) mod N] At[y][x] )_k,
-// fl = floor(N/2): the shift by s, the mask and ifftshift as one remap.
-template <int LOG2M>
-__global__ __launch_bounds__(fft::group_threads(LOG2M)) void pga_row_dft(
-    const float2 *__restrict__ At, int N, int Ny, const int *__restrict__ shift,
-    const float *__restrict__ mask, const float2 *__restrict__ ch2n,
-    const float2 *__restrict__ twM, const float2 *__restrict__ twN,
-    const float2 *__restrict__ H, float2 *__restrict__ G) {
-    using S = fft::Shape<LOG2M>;
-    constexpr int M = S::N, T = S::T, P = fft::rows_per_group(LOG2M), PAD = fft::padded(LOG2M);
-    __shared__ float2 lds[P * PAD];
-    const int lp = threadIdx.x / T, j = threadIdx.x % T;
-    const int y = blockIdx.x * P + lp;
-    const bool active = y < Ny;
-    float2 *Lr = lds + lp * PAD;
-    const int s = active ? shift[y] : 0;
-    const float2 *row = At + (size_t)(active ? y : 0) * N;
-    float2 v[16];
-#pragma unroll
-    for (int b = 0; b < S::NB0; b++)
-#pragma unroll
-        for (int r = 0; r < S::R0; r++) {
-            const int n = j + b * T + r * (M / S::R0);
-            float2 a = make_float2(0.f, 0.f);
-            if (active && n < N) {
-                int mi = n + s;
-                if (mi >= N) mi -= N;
-                const float m = mask[mi];
-                const float2 x = row[n];
-                a = cmul(make_float2(m * x.x, m * x.y), chirp(n, N, ch2n));
-            }
-            v[b * S::R0 + r] = conjf2(a);
-        }
-    bluestein_regs<LOG2M>(v, Lr, j, twM, H);
-    const float invM = 1.0f / (float)M;
-    const int fl = N / 2;
-    const long sh = ((long)(s - fl) % N + N) % N;
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int k = j + r * T;
-        if (active && k < N) {
-            float2 X = cmul(v[r], chirp(k, N, ch2n));
-            X.x *= invM;
-            X.y *= invM;
-            const float2 e = twN[((long)k * sh) % N];  // e^{+j 2 pi k sh / N}; use its conjugate
-            G[(size_t)y * N + k] = cmul(X, conjf2(e));
-        }
-    }
-}
-
-// Step 6 for rows y, in place: At[y] <- IDFT( DFT(At[y]) .* e^{-j phi} ),
-// the inverse as conj(DFT(conj(.)))/N through the same Bluestein step.
-template <int LOG2M>
-__global__ __launch_bounds__(fft::group_threads(LOG2M)) void pga_row_apply(
-    float2 *__restrict__ At, int N, int Ny, const double2 *__restrict__ ephi,
-    const float2 *__restrict__ ch2n, const float2 *__restrict__ twM,
-    const float2 *__restrict__ H) {
-    using S = fft::Shape<LOG2M>;
-    constexpr int M = S::N, T = S::T, P = fft::rows_per_group(LOG2M), PAD = fft::padded(LOG2M);
-    __shared__ float2 lds[P * PAD];
-    const int lp = threadIdx.x / T, j = threadIdx.x % T;
-    const int y = blockIdx.x * P + lp;
-    const bool active = y < Ny;
-    float2 *Lr = lds + lp * PAD;
-    float2 *row = At + (size_t)(active ? y : 0) * N;
-    float2 v[16];
-#pragma unroll
-    for (int b = 0; b < S::NB0; b++)
-#pragma unroll
-        for (int r = 0; r < S::R0; r++) {
-            const int n = j + b * T + r * (M / S::R0);
-            const float2 a = (active && n < N) ? cmul(row[n], chirp(n, N, ch2n))
-                                                : make_float2(0.f, 0.f);
-            v[b * S::R0 + r] = conjf2(a);
-        }
-    bluestein_regs<LOG2M>(v, Lr, j, twM, H);
-    const float invM = 1.0f / (float)M;
-    // Z_k = X_k e^{-j phi_k}; the next forward transform's input is
-    // conj(Z_k) w_k, zero-padded, fed conjugated in pass-0 order
-    float2 u[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int k = j + r * T;
-        float2 nx = make_float2(0.f, 0.f);
-        if (k < N) {
-            const float2 w = chirp(k, N, ch2n);
-            const float2 X = cmul(v[r], w);
-            const double2 e = ephi[k];
-            const float2 Z = cmul(make_float2(X.x * invM, X.y * invM),
-                                  make_float2((float)e.x, (float)e.y));
-            nx = cmul(conjf2(Z), w);
-        }
-        u[r] = conjf2(nx);
-    }
-    fft::to_input_order<LOG2M>(u, v);
-    bluestein_regs<LOG2M>(v, Lr, j, twM, H);
-    const float scale = invM / (float)N;
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int n = j + r * T;
-        if (active && n < N) {
-            const float2 uo = cmul(v[r], chirp(n, N, ch2n));
-            row[n] = make_float2(uo.x * scale, -uo.y * scale);
-        }
-    }
-}
-
-template <int LOG2M>
-void launch_rows(int Ny, bool apply, float2 *At, int N, const int *shift, const float *mask,
-                 const double2 *ephi, const float2 *ch2n, const float2 *twM, const float2 *twN,
-                 const float2 *H, float2 *G, hipStream_t s) {
-    constexpr int P = fft::rows_per_group(LOG2M);
-    const dim3 grid((Ny + P - 1) / P), block(fft::group_threads(LOG2M));
-    if (apply)
-        hipLaunchKernelGGL(pga_row_apply<LOG2M>, grid, block, 0, s, At, N, Ny, ephi, ch2n, twM, H);
-    else
-        hipLaunchKernelGGL(pga_row_dft<LOG2M>, grid, block, 0, s, At, N, Ny, shift, mask, ch2n,
-                           twM, twN, H, G);
-}
-
-// Row transforms (steps 3 and 6) for a transform length 2^L, 5 <= L <= 13:
-// one workgroup per row in LDS up to L = 9 (measured faster there), the
-// register-resident kernels from L = 10.
-int launch_row_kernel(int L, int Ny, bool apply, float2 *At, int N, const int *shift,
-                      const float *mask, const double2 *ephi, const float2 *ch2n,
-                      const float2 *twM, const float2 *twN, const float2 *H, float2 *G,
-                      hipStream_t s) {
-    if (L <= 9) {
-        const size_t lds = sizeof(float2) * ((size_t)1 << L);
-        if (apply)
-            hipLaunchKernelGGL(pga_row_apply_lds, dim3(Ny), dim3(kThreads), lds, s, At, N, L, ephi,
-                               ch2n, twM, H);
-        else
-            hipLaunchKernelGGL(pga_row_dft_lds, dim3(Ny), dim3(kThreads), lds, s, At, N, L, shift,
-                               mask, ch2n, twM, twN, H, G);
-        return SARF_OK;
-    }
-    switch (L) {
-#define SARF_PGA_ROWS(l) \
-        case l: launch_rows<l>(Ny, apply, At, N, shift, mask, ephi, ch2n, twM, twN, H, G, s); break;
-        SARF_PGA_ROWS(10) SARF_PGA_ROWS(11) SARF_PGA_ROWS(12) SARF_PGA_ROWS(13)
-#undef SARF_PGA_ROWS
-        default: return fail(SARF_ERR_UNSUPPORTED, "sarf_pga: transform length 2^%d", L);
-    }
-    return SARF_OK;
-}
-
-constexpr int kIntThreads = 1024;  // pga_integrate block; kPgaMaxN / 1024 = 4 per thread
 
 // numpy/MATLAB unwrap of one step d: d - 2 pi round-to-(-pi, pi] when |d| >= pi.
 __device__ inline double unwrap_step(double d) {
@@ -534,18 +314,14 @@ __device__ inline double block_sum_1024(double v, double *red) {
     return red[0];
 }
 
-// One workgroup of 1024: phi = unwrap([0 cumsum(phidot)]), detrend, rms,
-// af_ph += phi; writes e^{-j phi} (detrended) for the correction.
-// unwrap of a cumulative sum is the cumulative sum of the unwrapped steps,
-// so phi is one block-wide prefix scan of unwrap_step(phidot).
-__global__ __launch_bounds__(kIntThreads) void pga_integrate(const double *__restrict__ phidot,
-                                                             int Nx,
-                                                             double2 *__restrict__ ephi,
-                                                             double *__restrict__ af_ph,
-                                                             double *__restrict__ rms_out) {
+// One workgroup of kIntThreads: phi = unwrap([0 cumsum(phidot)]), detrend,
+// rms -> rms[it], af_ph += phi; writes e^{-j phi} (detrended) for the
+// correction.  unwrap of a cumulative sum is the cumulative sum of the
+// unwrapped steps, so phi is one block-wide prefix scan of unwrap_step.
+__device__ void integrate_body(const PgaImage &p, int it) {
     __shared__ double sc[kIntThreads];
     __shared__ double red[kIntThreads];
-    const int tid = threadIdx.x;
+    const int Nx = p.Nx, tid = threadIdx.x;
     const int per = (Nx + kIntThreads - 1) / kIntThreads;  // <= 4
     const int b = tid * per;
     double loc[4];
@@ -553,7 +329,7 @@ __global__ __launch_bounds__(kIntThreads) void pga_integrate(const double *__res
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int i = b + j;
-        if (j < per && i < Nx && i > 0) run += unwrap_step(phidot[i - 1]);
+        if (j < per && i < Nx && i > 0) run += unwrap_step(p.phidot[i - 1]);
         loc[j] = run;
     }
     sc[tid] = run;
@@ -588,20 +364,411 @@ __global__ __launch_bounds__(kIntThreads) void pga_integrate(const double *__res
         const int i = b + j;
         if (j < per && i < Nx) {
             const double v = loc[j] - (slope * i + icpt);
-            af_ph[i] += v;
+            p.af_ph[i] += v;
             double sn, cs;
             sincos(v, &sn, &cs);
-            ephi[i] = make_double2(cs, -sn);
+            p.ephi[i] = make_double2(cs, -sn);
             r += v * v;
         }
     }
     r = block_sum_1024(r, red);
-    if (tid == 0) *rms_out = sqrt(r / n);
+    if (tid == 0) p.rms[it] = sqrt(r / n);
 }
 
-__global__ void zero_f64(double *p, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0;
+// Short rows (M <= 512): one workgroup of kThreads per row, the row in LDS
+// (dynamic, M float2), radix-2 transforms -- more lanes per row than the
+// register-resident kernels give, so lower latency at these lengths.
+// Step 3 for row y:
+// G[y][k] = e^{-j 2 pi k (s - fl)/N} DFT_x( mask[(x + s) mod N] At[y][x] )_k,
+// fl = floor(N/2): the shift by s, the mask and ifftshift as one remap.
+__device__ void row_dft_lds_body(const PgaImage &p, int y) {
+    extern __shared__ float2 buf[];
+    const int N = p.Nx, L = p.L, M = 1 << L;
+    const int s = p.shift[y];
+    const float2 *row = p.At + (size_t)y * N;
+    for (int x = threadIdx.x; x < M; x += kThreads) {
+        float2 v = make_float2(0.f, 0.f);
+        if (x < N) {
+            int mi = x + s;
+            if (mi >= N) mi -= N;
+            const float m = p.mask[mi];
+            const float2 a = row[x];
+            v = cmul(make_float2(m * a.x, m * a.y), chirp(x, N, p.ch2n));
+        }
+        buf[x] = v;
+    }
+    __syncthreads();
+    bluestein_core(buf, L, p.twM, p.H);
+    const float invM = 1.0f / (float)M;
+    const int fl = N / 2;
+    const long sh = ((long)(s - fl) % N + N) % N;
+    for (int k = threadIdx.x; k < N; k += kThreads) {
+        float2 X = cmul(buf[k], chirp(k, N, p.ch2n));
+        X.x *= invM;
+        X.y *= invM;
+        const float2 e = p.twN[((long)k * sh) % N];  // e^{+j 2 pi k sh / N}; use its conjugate
+        p.G[(size_t)y * N + k] = cmul(X, conjf2(e));
+    }
+}
+
+// Step 6 for row y, in place: At[y] <- IDFT( DFT(At[y]) .* e^{-j phi} ),
+// the inverse as conj(DFT(conj(.)))/N through the same Bluestein core.
+__device__ void row_apply_lds_body(const PgaImage &p, int y) {
+    extern __shared__ float2 buf[];
+    const int N = p.Nx, L = p.L, M = 1 << L;
+    float2 *row = p.At + (size_t)y * N;
+    for (int x = threadIdx.x; x < M; x += kThreads)
+        buf[x] = x < N ? cmul(row[x], chirp(x, N, p.ch2n)) : make_float2(0.f, 0.f);
+    __syncthreads();
+    bluestein_core(buf, L, p.twM, p.H);
+    const float invM = 1.0f / (float)M;
+    // Z_k = X_k e^{-j phi_k}; next input conj(Z_k) w_k, zero-padded (each
+    // lane reads and rewrites only its own elements)
+    for (int k = threadIdx.x; k < M; k += kThreads) {
+        float2 v = make_float2(0.f, 0.f);
+        if (k < N) {
+            const float2 w = chirp(k, N, p.ch2n);
+            const float2 X = cmul(buf[k], w);
+            const double2 e = p.ephi[k];
+            const float2 Z = cmul(make_float2(X.x * invM, X.y * invM),
+                                  make_float2((float)e.x, (float)e.y));
+            v = cmul(conjf2(Z), w);
+        }
+        buf[k] = v;
+    }
+    __syncthreads();
+    bluestein_core(buf, L, p.twM, p.H);
+    const float scale = invM / (float)N;
+    for (int n = threadIdx.x; n < N; n += kThreads) {
+        const float2 u = cmul(buf[n], chirp(n, N, p.ch2n));
+        row[n] = make_float2(u.x * scale, -u.y * scale);
+    }
+}
+
+// Bluestein's middle step on the forward transform of the chirped row held
+// in pass-0 order (v = conj of the input, as ifft_regs computes the forward
+// transform as conj(ifft(conj))): multiply by H and run the inverse
+// transform; on return v[r] = b_k (k = j + r*T) and X_k = w_k b_k / M.
+template <int LOG2M>
+__device__ __forceinline__ void bluestein_regs(float2 (&v)[16], float2 *Lr, int j,
+                                               const float2 *__restrict__ twM,
+                                               const float2 *__restrict__ H) {
+    constexpr int T = fft::Shape<LOG2M>::T;
+    fft::ifft_regs<LOG2M>(v, Lr, j, twM);  // v[r] = conj(A_k)
+    float2 u[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) u[r] = cmul(conjf2(v[r]), H[j + r * T]);
+    fft::to_input_order<LOG2M>(u, v);
+    fft::ifft_regs<LOG2M>(v, Lr, j, twM);
+}
+
+// Step 3 for the P rows of block blk (M/16 lanes per row, M = 2^LOG2M), as
+// row_dft_lds_body.
+template <int LOG2M>
+__device__ __forceinline__ void row_dft_body(const PgaImage &p, int blk) {
+    using S = fft::Shape<LOG2M>;
+    constexpr int M = S::N, T = S::T, P = fft::rows_per_group(LOG2M), PAD = fft::padded(LOG2M);
+    __shared__ float2 lds[P * PAD];
+    const int N = p.Nx;
+    const int lp = threadIdx.x / T, j = threadIdx.x % T;
+    const int y = blk * P + lp;
+    const bool active = y < p.Ny;
+    float2 *Lr = lds + lp * PAD;
+    const int s = active ? p.shift[y] : 0;
+    const float2 *row = p.At + (size_t)(active ? y : 0) * N;
+    float2 v[16];
+#pragma unroll
+    for (int b = 0; b < S::NB0; b++)
+#pragma unroll
+        for (int r = 0; r < S::R0; r++) {
+            const int n = j + b * T + r * (M / S::R0);
+            float2 a = make_float2(0.f, 0.f);
+            if (active && n < N) {
+                int mi = n + s;
+                if (mi >= N) mi -= N;
+                const float m = p.mask[mi];
+                const float2 x = row[n];
+                a = cmul(make_float2(m * x.x, m * x.y), chirp(n, N, p.ch2n));
+            }
+            v[b * S::R0 + r] = conjf2(a);
+        }
+    bluestein_regs<LOG2M>(v, Lr, j, p.twM, p.H);
+    const float invM = 1.0f / (float)M;
+    const int fl = N / 2;
+    const long sh = ((long)(s - fl) % N + N) % N;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int k = j + r * T;
+        if (active && k < N) {
+            float2 X = cmul(v[r], chirp(k, N, p.ch2n));
+            X.x *= invM;
+            X.y *= invM;
+            const float2 e = p.twN[((long)k * sh) % N];  // e^{+j 2 pi k sh / N}; use its conjugate
+            p.G[(size_t)y * N + k] = cmul(X, conjf2(e));
+        }
+    }
+}
+
+// Step 6 for the P rows of block blk, in place, as row_apply_lds_body.
+template <int LOG2M>
+__device__ __forceinline__ void row_apply_body(const PgaImage &p, int blk) {
+    using S = fft::Shape<LOG2M>;
+    constexpr int M = S::N, T = S::T, P = fft::rows_per_group(LOG2M), PAD = fft::padded(LOG2M);
+    __shared__ float2 lds[P * PAD];
+    const int N = p.Nx;
+    const int lp = threadIdx.x / T, j = threadIdx.x % T;
+    const int y = blk * P + lp;
+    const bool active = y < p.Ny;
+    float2 *Lr = lds + lp * PAD;
+    float2 *row = p.At + (size_t)(active ? y : 0) * N;
+    float2 v[16];
+#pragma unroll
+    for (int b = 0; b < S::NB0; b++)
+#pragma unroll
+        for (int r = 0; r < S::R0; r++) {
+            const int n = j + b * T + r * (M / S::R0);
+            const float2 a = (active && n < N) ? cmul(row[n], chirp(n, N, p.ch2n))
+                                                : make_float2(0.f, 0.f);
+            v[b * S::R0 + r] = conjf2(a);
+        }
+    bluestein_regs<LOG2M>(v, Lr, j, p.twM, p.H);
+    const float invM = 1.0f / (float)M;
+    // Z_k = X_k e^{-j phi_k}; the next forward transform's input is
+    // conj(Z_k) w_k, zero-padded, fed conjugated in pass-0 order
+    float2 u[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int k = j + r * T;
+        float2 nx = make_float2(0.f, 0.f);
+        if (k < N) {
+            const float2 w = chirp(k, N, p.ch2n);
+            const float2 X = cmul(v[r], w);
+            const double2 e = p.ephi[k];
+            const float2 Z = cmul(make_float2(X.x * invM, X.y * invM),
+                                  make_float2((float)e.x, (float)e.y));
+            nx = cmul(conjf2(Z), w);
+        }
+        u[r] = conjf2(nx);
+    }
+    fft::to_input_order<LOG2M>(u, v);
+    bluestein_regs<LOG2M>(v, Lr, j, p.twM, p.H);
+    const float scale = invM / (float)N;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int n = j + r * T;
+        if (active && n < N) {
+            const float2 uo = cmul(v[r], chirp(n, N, p.ch2n));
+            row[n] = make_float2(uo.x * scale, -uo.y * scale);
+        }
+    }
+}
+
+// Blocks of a row step for an image of Ny rows: one workgroup per row up to
+// L = 9, rows_per_group rows per workgroup from L = 10.
+int row_blocks(int L, int Ny) {
+    if (L <= 9) return Ny;
+    const int P = fft::rows_per_group(L);
+    return (Ny + P - 1) / P;
+}
+
+// ------------------------------------------------- kernels: one image
+
+__global__ __launch_bounds__(kThreads) void k1_setup(PgaImage p, int spectrum) {
+    setup_body(p, spectrum != 0);
+}
+__global__ __launch_bounds__(kThreads) void k1_transpose(PgaImage p, int forward) {
+    if (forward) transpose_body(p.img, p.Nx, p.Ny, p.At, blockIdx.x, blockIdx.y);
+    else transpose_body(p.At, p.Ny, p.Nx, p.img, blockIdx.x, blockIdx.y);
+}
+__global__ __launch_bounds__(kThreads) void k1_argmax(PgaImage p) { argmax_body(p, blockIdx.x); }
+__global__ __launch_bounds__(kThreads) void k1_col_partial(PgaImage p, int mode) {
+    col_partial_body(p, mode, blockIdx.x, blockIdx.y);
+}
+__global__ __launch_bounds__(kThreads) void k1_col_final(PgaImage p, int mode) {
+    col_final_body(p, mode, blockIdx.x);
+}
+__global__ __launch_bounds__(kIntThreads) void k1_window(PgaImage p, double db_down) {
+    window_body(p, db_down);
+}
+__global__ __launch_bounds__(kIntThreads) void k1_integrate(PgaImage p, int it) {
+    integrate_body(p, it);
+}
+__global__ __launch_bounds__(kThreads) void k1_rows_lds(PgaImage p, int apply) {
+    if (apply) row_apply_lds_body(p, blockIdx.x);
+    else row_dft_lds_body(p, blockIdx.x);
+}
+template <int LOG2M>
+__global__ __launch_bounds__(fft::group_threads(LOG2M)) void k1_rows(PgaImage p, int apply) {
+    if (apply) row_apply_body<LOG2M>(p, blockIdx.x);
+    else row_dft_body<LOG2M>(p, blockIdx.x);
+}
+
+// ------------------------------------------------ kernels: many images
+
+// Image i owns blocks [off[i], off[i+1]) of a launch over n images.
+__device__ __forceinline__ int owner(const int *__restrict__ off, int n, int blk) {
+    int lo = 0, hi = n;  // off[lo] <= blk < off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= blk) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Per-image block offsets, one array of n + 1 per step kind.
+enum OffKind { kOffTranspose = 0, kOffRows, kOffColPartial, kOffColFinal, kOffKinds };
+
+struct PgaBatchDev {
+    const PgaImage *img;  // n images
+    const int *off;       // kOffKinds x (n + 1)
+    const int *grp;       // row-step groups: image ids, then block offsets
+    int n;
+};
+
+__global__ __launch_bounds__(kThreads) void kb_setup(PgaBatchDev b, int spectrum) {
+    setup_body(b.img[blockIdx.x], spectrum != 0);
+}
+__global__ __launch_bounds__(kThreads) void kb_transpose(PgaBatchDev b, int forward) {
+    const int *off = b.off + kOffTranspose * (b.n + 1);
+    const int i = owner(off, b.n, blockIdx.x);
+    const PgaImage &p = b.img[i];
+    const int l = blockIdx.x - off[i];
+    const int R = forward ? p.Nx : p.Ny, C = forward ? p.Ny : p.Nx;
+    const int gx = (C + 63) / 64;
+    if (forward) transpose_body(p.img, R, C, p.At, l % gx, l / gx);
+    else transpose_body(p.At, R, C, p.img, l % gx, l / gx);
+}
+__global__ __launch_bounds__(kThreads) void kb_argmax(PgaBatchDev b) {
+    const int *off = b.off + kOffRows * (b.n + 1);
+    const int i = owner(off, b.n, blockIdx.x);
+    argmax_body(b.img[i], blockIdx.x - off[i]);
+}
+__global__ __launch_bounds__(kThreads) void kb_col_partial(PgaBatchDev b, int mode) {
+    const int *off = b.off + kOffColPartial * (b.n + 1);
+    const int i = owner(off, b.n, blockIdx.x);
+    const PgaImage &p = b.img[i];
+    const int l = blockIdx.x - off[i], gx = (p.Nx + 63) / 64;
+    col_partial_body(p, mode, l % gx, l / gx);
+}
+__global__ __launch_bounds__(kThreads) void kb_col_final(PgaBatchDev b, int mode) {
+    const int *off = b.off + kOffColFinal * (b.n + 1);
+    const int i = owner(off, b.n, blockIdx.x);
+    col_final_body(b.img[i], mode, blockIdx.x - off[i]);
+}
+__global__ __launch_bounds__(kIntThreads) void kb_window(PgaBatchDev b, double db_down) {
+    window_body(b.img[blockIdx.x], db_down);
+}
+__global__ __launch_bounds__(kIntThreads) void kb_integrate(PgaBatchDev b, int it) {
+    integrate_body(b.img[blockIdx.x], it);
+}
+// Row step of one transform length: grp[g0 .. g0 + m) are the images' ids,
+// grp[g0 + m .. g0 + 2m] their block offsets.
+__global__ __launch_bounds__(kThreads) void kb_rows_lds(PgaBatchDev b, int g0, int m, int apply) {
+    const int *ids = b.grp + g0, *off = ids + m;
+    const int k = owner(off, m, blockIdx.x);
+    const PgaImage &p = b.img[ids[k]];
+    if (apply) row_apply_lds_body(p, blockIdx.x - off[k]);
+    else row_dft_lds_body(p, blockIdx.x - off[k]);
+}
+template <int LOG2M>
+__global__ __launch_bounds__(fft::group_threads(LOG2M)) void kb_rows(PgaBatchDev b, int g0, int m,
+                                                                    int apply) {
+    const int *ids = b.grp + g0, *off = ids + m;
+    const int k = owner(off, m, blockIdx.x);
+    const PgaImage &p = b.img[ids[k]];
+    if (apply) row_apply_body<LOG2M>(p, blockIdx.x - off[k]);
+    else row_dft_body<LOG2M>(p, blockIdx.x - off[k]);
+}
+
+// ---------------------------------------------------------------- host
+
+int transform_log2(int N) {
+    int L = kMinL;  // the row kernels take 32 <= M <= 8192
+    while ((1 << L) < 2 * N - 1) L++;
+    return L;
+}
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// Bytes of one image's small state, laid out by carve().
+size_t small_bytes(int Nx, int Ny, int L) {
+    return align256(sizeof(int) * Ny) + 2 * align256(sizeof(double) * kRowChunks * Nx) +
+           2 * align256(sizeof(double) * Nx) + align256(sizeof(double2) * Nx) +
+           align256(sizeof(float) * Nx) + align256(sizeof(float2) * ((size_t)1 << L));
+}
+
+void carve(PgaImage &p, char *base) {
+    auto take = [&](size_t bytes) {
+        char *r = base;
+        base += align256(bytes);
+        return r;
+    };
+    p.shift = (int *)take(sizeof(int) * p.Ny);
+    p.pa = (double *)take(sizeof(double) * kRowChunks * p.Nx);
+    p.pb = (double *)take(sizeof(double) * kRowChunks * p.Nx);
+    p.w = (double *)take(sizeof(double) * p.Nx);
+    p.phidot = (double *)take(sizeof(double) * p.Nx);
+    p.ephi = (double2 *)take(sizeof(double2) * p.Nx);
+    p.mask = (float *)take(sizeof(float) * p.Nx);
+    p.H = (float2 *)take(sizeof(float2) * ((size_t)1 << p.L));
+}
+
+int tables(sarf_ctx *ctx, PgaImage &p) {
+    SARF_TRY(twiddles_any_f32(ctx, 1L << p.L, &p.twM));
+    SARF_TRY(twiddles_any_f32(ctx, p.Nx, &p.twN));
+    SARF_TRY(twiddles_any_f32(ctx, 2L * p.Nx, &p.ch2n));
+    return SARF_OK;
+}
+
+template <int LOG2M>
+void launch_k1_rows(const PgaImage &p, int apply, hipStream_t s) {
+    hipLaunchKernelGGL(k1_rows<LOG2M>, dim3(row_blocks(LOG2M, p.Ny)),
+                       dim3(fft::group_threads(LOG2M)), 0, s, p, apply);
+}
+
+int k1_row_step(const PgaImage &p, int apply, hipStream_t s) {
+    if (p.L <= 9) {
+        hipLaunchKernelGGL(k1_rows_lds, dim3(p.Ny), dim3(kThreads),
+                           sizeof(float2) * ((size_t)1 << p.L), s, p, apply);
+        return SARF_OK;
+    }
+    switch (p.L) {
+        case 10: launch_k1_rows<10>(p, apply, s); break;
+        case 11: launch_k1_rows<11>(p, apply, s); break;
+        case 12: launch_k1_rows<12>(p, apply, s); break;
+        case 13: launch_k1_rows<13>(p, apply, s); break;
+        default: return fail(SARF_ERR_UNSUPPORTED, "sarf_pga: transform length 2^%d", p.L);
+    }
+    return SARF_OK;
+}
+
+template <int LOG2M>
+void launch_kb_rows(const PgaBatchDev &b, int g0, int m, int blocks, int apply, hipStream_t s) {
+    hipLaunchKernelGGL(kb_rows<LOG2M>, dim3(blocks), dim3(fft::group_threads(LOG2M)), 0, s, b,
+                       g0, m, apply);
+}
+
+int kb_row_step(const PgaBatchDev &b, int L, int g0, int m, int blocks, int apply, hipStream_t s) {
+    if (L <= 9) {
+        hipLaunchKernelGGL(kb_rows_lds, dim3(blocks), dim3(kThreads),
+                           sizeof(float2) * ((size_t)1 << L), s, b, g0, m, apply);
+        return SARF_OK;
+    }
+    switch (L) {
+        case 10: launch_kb_rows<10>(b, g0, m, blocks, apply, s); break;
+        case 11: launch_kb_rows<11>(b, g0, m, blocks, apply, s); break;
+        case 12: launch_kb_rows<12>(b, g0, m, blocks, apply, s); break;
+        case 13: launch_kb_rows<13>(b, g0, m, blocks, apply, s); break;
+        default: return fail(SARF_ERR_UNSUPPORTED, "sarf_pga_scenes: transform length 2^%d", L);
+    }
+    return SARF_OK;
+}
+
+int check_image(const char *who, int Nx, int Ny) {
+    if (Nx < 2 || Ny < 1) return fail(SARF_ERR_INVALID, "%s: Nx=%d Ny=%d", who, Nx, Ny);
+    if (Nx > kPgaMaxN) return fail(SARF_ERR_UNSUPPORTED, "%s: Nx=%d > %d", who, Nx, kPgaMaxN);
+    return SARF_OK;
 }
 
 }  // namespace
@@ -614,75 +781,182 @@ extern "C" int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *d, void *img, double
     clear_error();
     SARF_TRY(use_device(ctx));
     if (!d || !img || !af_ph || !rms) return fail(SARF_ERR_INVALID, "sarf_pga: null argument");
-    const int Nx = d->Nx, Ny = d->Ny;
-    if (Nx < 2 || Ny < 1 || d->iters < 0)
-        return fail(SARF_ERR_INVALID, "sarf_pga: Nx=%d Ny=%d iters=%d", Nx, Ny, d->iters);
-    if (Nx > kPgaMaxN)
-        return fail(SARF_ERR_UNSUPPORTED, "sarf_pga: Nx=%d > %d", Nx, kPgaMaxN);
+    SARF_TRY(check_image("sarf_pga", d->Nx, d->Ny));
+    if (d->iters < 0) return fail(SARF_ERR_INVALID, "sarf_pga: iters=%d", d->iters);
     hipStream_t s = (hipStream_t)stream;
-    const int N = Nx;
-    int L = 5;  // the row kernels take 32 <= M <= 8192
-    while ((1 << L) < 2 * N - 1) L++;
-    const int M = 1 << L;  // <= 8192
-    const size_t npix = (size_t)Nx * Ny;
-    const size_t lds = sizeof(float2) * (size_t)M;
+    PgaImage p{};
+    p.Nx = d->Nx;
+    p.Ny = d->Ny;
+    p.L = transform_log2(p.Nx);
+    p.img = (float2 *)img;
+    p.af_ph = af_ph;
+    p.rms = rms;
+    const size_t npix = (size_t)p.Nx * p.Ny;
     void *bufA, *bufG, *bufS;
     SARF_TRY(scratch_get(ctx, "pga.At", sizeof(float2) * npix, &bufA));
     SARF_TRY(scratch_get(ctx, "pga.G", sizeof(float2) * npix, &bufG));
-    auto align16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
-    const size_t o_parta = align16(sizeof(int) * Ny);
-    const size_t o_partb = o_parta + sizeof(double) * kRowChunks * N;
-    const size_t o_w = o_partb + sizeof(double) * kRowChunks * N;
-    const size_t o_phidot = o_w + sizeof(double) * N;
-    const size_t o_ephi = o_phidot + sizeof(double) * N;
-    const size_t o_mask = o_ephi + sizeof(double2) * N;
-    const size_t o_H = o_mask + align16(sizeof(float) * N);
-    const size_t small = o_H + sizeof(float2) * M;
-    SARF_TRY(scratch_get(ctx, "pga.small", small, &bufS));
-    char *base = (char *)bufS;
-    int *shift = (int *)base;
-    double *pa = (double *)(base + o_parta);
-    double *pb = (double *)(base + o_partb);
-    double *w = (double *)(base + o_w);
-    double *phidot = (double *)(base + o_phidot);
-    double2 *ephi = (double2 *)(base + o_ephi);
-    float *mask = (float *)(base + o_mask);
-    float2 *H = (float2 *)(base + o_H);
-    const float2 *twM = nullptr, *twN = nullptr, *ch2n = nullptr;
-    SARF_TRY(twiddles_any_f32(ctx, M, &twM));
-    SARF_TRY(twiddles_any_f32(ctx, N, &twN));
-    SARF_TRY(twiddles_any_f32(ctx, 2L * N, &ch2n));
-    float2 *A = (float2 *)img, *At = (float2 *)bufA, *G = (float2 *)bufG;
+    SARF_TRY(scratch_get(ctx, "pga.small", small_bytes(p.Nx, p.Ny, p.L), &bufS));
+    p.At = (float2 *)bufA;
+    p.G = (float2 *)bufG;
+    carve(p, (char *)bufS);
+    SARF_TRY(tables(ctx, p));
+    const size_t lds = sizeof(float2) * ((size_t)1 << p.L);
+    hipLaunchKernelGGL(k1_setup, dim3(1), dim3(kThreads), lds, s, p, d->iters > 0 ? 1 : 0);
+    if (d->iters > 0) {
+        // img is Nx rows (x) of Ny values; At is Ny rows of Nx values
+        hipLaunchKernelGGL(k1_transpose, dim3((p.Ny + 63) / 64, (p.Nx + 63) / 64), dim3(kThreads),
+                           0, s, p, 1);
+        const dim3 gcp((p.Nx + 63) / 64, kRowChunks), gcf((p.Nx + kThreads - 1) / kThreads);
+        for (int it = 0; it < d->iters; it++) {
+            hipLaunchKernelGGL(k1_argmax, dim3(p.Ny), dim3(kThreads), 0, s, p);
+            hipLaunchKernelGGL(k1_col_partial, gcp, dim3(kThreads), 0, s, p, 0);
+            hipLaunchKernelGGL(k1_col_final, gcf, dim3(kThreads), 0, s, p, 0);
+            hipLaunchKernelGGL(k1_window, dim3(1), dim3(kIntThreads), 0, s, p, d->db_down);
+            SARF_TRY(k1_row_step(p, 0, s));
+            hipLaunchKernelGGL(k1_col_partial, gcp, dim3(kThreads), 0, s, p, 1);
+            hipLaunchKernelGGL(k1_col_final, gcf, dim3(kThreads), 0, s, p, 1);
+            hipLaunchKernelGGL(k1_integrate, dim3(1), dim3(kIntThreads), 0, s, p, it);
+            SARF_TRY(k1_row_step(p, 1, s));
+            SARF_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k1_transpose, dim3((p.Nx + 63) / 64, (p.Ny + 63) / 64), dim3(kThreads),
+                           0, s, p, 0);
+    }
+    SARF_HIP(hipGetLastError());
+    return SARF_OK;
+}
 
-    hipLaunchKernelGGL(zero_f64, dim3((Nx + 255) / 256), dim3(256), 0, s, af_ph, Nx);
-    if (d->iters == 0) {
-        SARF_HIP(hipGetLastError());
-        return SARF_OK;
+extern "C" int sarf_pga_scenes(sarf_ctx *ctx, int nscenes, const sarf_pga_scene *scenes, int iters,
+                               double db_down, void *stream) {
+    clear_error();
+    SARF_TRY(use_device(ctx));
+    if (nscenes < 1 || !scenes || iters < 0)
+        return fail(SARF_ERR_INVALID, "sarf_pga_scenes: nscenes=%d iters=%d", nscenes, iters);
+    const int n = nscenes;
+    std::vector<PgaImage> host(n);  // value-initialised: padding is zero (the cache key)
+    size_t tot_pix = 0, tot_small = 0;
+    for (int i = 0; i < n; i++) {
+        const sarf_pga_scene &c = scenes[i];
+        if (!c.img || !c.af_ph || !c.rms)
+            return fail(SARF_ERR_INVALID, "sarf_pga_scenes: scene %d has a null pointer", i);
+        SARF_TRY(check_image("sarf_pga_scenes", c.Nx, c.Ny));
+        PgaImage &p = host[i];
+        p.Nx = c.Nx;
+        p.Ny = c.Ny;
+        p.L = transform_log2(c.Nx);
+        p.img = (float2 *)c.img;
+        p.af_ph = c.af_ph;
+        p.rms = c.rms;
+        tot_pix += align256(sizeof(float2) * (size_t)c.Nx * c.Ny);
+        tot_small += small_bytes(p.Nx, p.Ny, p.L);
+        SARF_TRY(tables(ctx, p));
     }
-    hipLaunchKernelGGL(pga_chirp_spectrum, dim3(1), dim3(kThreads), lds, s, N, L, ch2n, twM, H);
-    // img is Nx rows (x) of Ny values; At is Ny rows of Nx values
-    hipLaunchKernelGGL(transpose_c64, dim3((Ny + 63) / 64, (Nx + 63) / 64), dim3(kThreads), 0, s,
-                       A, Nx, Ny, At);
-    const dim3 gcol((N + 63) / 64, kRowChunks);
-    for (int it = 0; it < d->iters; it++) {
-        hipLaunchKernelGGL(pga_row_argmax, dim3(Ny), dim3(kThreads), 0, s, At, N, shift);
-        hipLaunchKernelGGL(pga_col_partial, gcol, dim3(kThreads), 0, s, At, N, Ny, 0, pa, pb);
-        hipLaunchKernelGGL(pga_col_final, dim3((N + kThreads - 1) / kThreads), dim3(kThreads), 0, s,
-                           pa, pb, N, 0, w);
-        hipLaunchKernelGGL(pga_window, dim3(1), dim3(1024), 0, s, w, N, d->db_down, mask);
-        SARF_TRY(launch_row_kernel(L, Ny, false, At, N, shift, mask, ephi, ch2n, twM, twN, H, G,
-                                   s));
-        hipLaunchKernelGGL(pga_col_partial, gcol, dim3(kThreads), 0, s, G, N, Ny, 1, pa, pb);
-        hipLaunchKernelGGL(pga_col_final, dim3((N + kThreads - 1) / kThreads), dim3(kThreads), 0, s,
-                           pa, pb, N, 1, phidot);
-        hipLaunchKernelGGL(pga_integrate, dim3(1), dim3(kIntThreads), 0, s, phidot, N, ephi, af_ph,
-                           rms + it);
-        SARF_TRY(launch_row_kernel(L, Ny, true, At, N, shift, mask, ephi, ch2n, twM, twN, H, G,
-                                   s));
-        SARF_HIP(hipGetLastError());
+    void *bufA, *bufG, *bufS;
+    SARF_TRY(scratch_get(ctx, "pgab.At", tot_pix, &bufA));
+    SARF_TRY(scratch_get(ctx, "pgab.G", tot_pix, &bufG));
+    SARF_TRY(scratch_get(ctx, "pgab.small", tot_small, &bufS));
+    {
+        size_t op = 0, os = 0;
+        for (int i = 0; i < n; i++) {
+            PgaImage &p = host[i];
+            p.At = (float2 *)((char *)bufA + op);
+            p.G = (float2 *)((char *)bufG + op);
+            op += align256(sizeof(float2) * (size_t)p.Nx * p.Ny);
+            carve(p, (char *)bufS + os);
+            os += small_bytes(p.Nx, p.Ny, p.L);
+        }
     }
-    hipLaunchKernelGGL(transpose_c64, dim3((Nx + 63) / 64, (Ny + 63) / 64), dim3(kThreads), 0, s,
-                       At, Ny, Nx, A);
+    // block offsets per step kind, and the row-step groups by transform length
+    std::vector<int> off((size_t)kOffKinds * (n + 1), 0);
+    for (int i = 0; i < n; i++) {
+        const PgaImage &p = host[i];
+        const int cnt[kOffKinds] = {((p.Nx + 63) / 64) * ((p.Ny + 63) / 64), p.Ny,
+                                    ((p.Nx + 63) / 64) * kRowChunks,
+                                    (p.Nx + kThreads - 1) / kThreads};
+        for (int k = 0; k < kOffKinds; k++)
+            off[(size_t)k * (n + 1) + i + 1] = off[(size_t)k * (n + 1) + i] + cnt[k];
+    }
+    struct Group {
+        int L, g0, m, blocks;
+    };
+    std::vector<Group> groups;
+    std::vector<int> grp;
+    int maxL = kMinL;
+    for (int L = kMinL; L <= kMaxL; L++) {
+        std::vector<int> ids;
+        for (int i = 0; i < n; i++)
+            if (host[i].L == L) ids.push_back(i);
+        if (ids.empty()) continue;
+        maxL = L;
+        Group g{L, (int)grp.size(), (int)ids.size(), 0};
+        grp.insert(grp.end(), ids.begin(), ids.end());
+        grp.push_back(0);
+        for (int id : ids) {
+            g.blocks += row_blocks(L, host[id].Ny);
+            grp.push_back(g.blocks);
+        }
+        groups.push_back(g);
+    }
+    const size_t b_img = sizeof(PgaImage) * (size_t)n, b_off = sizeof(int) * off.size(),
+                 b_grp = sizeof(int) * grp.size();
+    void *buf = nullptr;
+    SARF_TRY(scratch_get(ctx, "pgab.desc", align256(b_img) + align256(b_off) + b_grp, &buf));
+    PgaBatchDev b{};
+    b.img = (const PgaImage *)buf;
+    b.off = (const int *)((char *)buf + align256(b_img));
+    b.grp = (const int *)((char *)buf + align256(b_img) + align256(b_off));
+    b.n = n;
+    hipStream_t s = (hipStream_t)stream;
+    // The device state depends only on the caller's scenes and the scratch
+    // buffers: when it matches the last upload into this buffer, launch
+    // again without copying (and without waiting, so the call can be
+    // captured into a graph once the state has been uploaded outside it).
+    std::vector<unsigned char> key(b_img + b_off + b_grp);
+    std::memcpy(key.data(), host.data(), b_img);
+    std::memcpy(key.data() + b_img, off.data(), b_off);
+    std::memcpy(key.data() + b_img + b_off, grp.data(), b_grp);
+    bool cached;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        cached = ctx->pga_buf == buf && ctx->pga_key == key;
+    }
+    if (!cached) {
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        SARF_HIP(hipStreamIsCapturing(s, &cap));
+        if (cap != hipStreamCaptureStatusNone)
+            return fail(SARF_ERR_INVALID,
+                        "sarf_pga_scenes: a new scene list cannot be uploaded during stream "
+                        "capture; make the same call once before capturing");
+        // pageable sources: the copies complete before returning
+        SARF_HIP(hipMemcpyAsync((void *)b.img, host.data(), b_img, hipMemcpyHostToDevice, s));
+        SARF_HIP(hipMemcpyAsync((void *)b.off, off.data(), b_off, hipMemcpyHostToDevice, s));
+        SARF_HIP(hipMemcpyAsync((void *)b.grp, grp.data(), b_grp, hipMemcpyHostToDevice, s));
+        SARF_HIP(hipStreamSynchronize(s));
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->pga_buf = buf;
+        ctx->pga_key = std::move(key);
+    }
+    const auto total = [&](int kind) { return off[(size_t)kind * (n + 1) + n]; };
+    const size_t lds = sizeof(float2) * ((size_t)1 << maxL);
+    hipLaunchKernelGGL(kb_setup, dim3(n), dim3(kThreads), lds, s, b, iters > 0 ? 1 : 0);
+    if (iters > 0) {
+        hipLaunchKernelGGL(kb_transpose, dim3(total(kOffTranspose)), dim3(kThreads), 0, s, b, 1);
+        for (int it = 0; it < iters; it++) {
+            hipLaunchKernelGGL(kb_argmax, dim3(total(kOffRows)), dim3(kThreads), 0, s, b);
+            hipLaunchKernelGGL(kb_col_partial, dim3(total(kOffColPartial)), dim3(kThreads), 0, s,
+                               b, 0);
+            hipLaunchKernelGGL(kb_col_final, dim3(total(kOffColFinal)), dim3(kThreads), 0, s, b, 0);
+            hipLaunchKernelGGL(kb_window, dim3(n), dim3(kIntThreads), 0, s, b, db_down);
+            for (const Group &g : groups) SARF_TRY(kb_row_step(b, g.L, g.g0, g.m, g.blocks, 0, s));
+            hipLaunchKernelGGL(kb_col_partial, dim3(total(kOffColPartial)), dim3(kThreads), 0, s,
+                               b, 1);
+            hipLaunchKernelGGL(kb_col_final, dim3(total(kOffColFinal)), dim3(kThreads), 0, s, b, 1);
+            hipLaunchKernelGGL(kb_integrate, dim3(n), dim3(kIntThreads), 0, s, b, it);
+            for (const Group &g : groups) SARF_TRY(kb_row_step(b, g.L, g.g0, g.m, g.blocks, 1, s));
+            SARF_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(kb_transpose, dim3(total(kOffTranspose)), dim3(kThreads), 0, s, b, 0);
+    }
     SARF_HIP(hipGetLastError());
     return SARF_OK;
 }
