@@ -100,23 +100,26 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
     T.init(a, t);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
-    // per-pixel forward image and entropy weight 2 w_p
+    // per-pixel forward image times the entropy weight 2 w_p and times
+    // invbin: the profile slope below is per bin, dv/dR = invbin * slope, so
+    // z = v' + j k v is formed per bin as (slope + j (k/invbin) v) and the
+    // factor invbin rides on these weights
+    const float invbin = a.invbin;
     const double E = ent[0], S = ent[1];
     const float invS = (float)(1.0 / S), lnS = (float)log(S), Ef = (float)E;
-    float Ir[PX], Ii[PX], wg[PX];
+    float WI[PX], WQ[PX];
 #pragma unroll
     for (int j = 0; j < PX; j++) {
         int x, y;
-        Ir[j] = Ii[j] = wg[j] = 0.f;
+        WI[j] = WQ[j] = 0.f;
         if (T.pixel(j, x, y, a)) {
             const float2 v = img[(size_t)x * a.Ny + y];
             const float m = v.x * v.x + v.y * v.y;
-            Ir[j] = v.x;
-            Ii[j] = v.y;
-            wg[j] = m > 0.f ? -2.f * (__logf(m) - lnS + Ef) * invS : 0.f;
+            const float wg = m > 0.f ? -2.f * (__logf(m) - lnS + Ef) * invS : 0.f;
+            WI[j] = wg * invbin * v.x;
+            WQ[j] = wg * invbin * v.y;
         }
     }
-    const float invbin = a.invbin;
     const float two_pi = 6.283185307179586f;
     const PlainPos pos{ant};
 
@@ -145,34 +148,62 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
                     const float up = XL ? A.y : A.x;  // uh along the pixel axis
                     const float sy = ul * T.qy;
                     const float wmax = B.w;
-                    const float kk = two_pi * B.z;  // rad per metre
-                    const float glo = (float)(-I.x), ghi = (float)(a.Nfft - 1 - I.x);
+                    const float kb = two_pi * B.z / invbin;  // rad per bin
                     float sA = 0.f, sB = 0.f, sC = 0.f;
+                    if (I.w) {
+                        // fast pulse (see the forward kernel): no gate, no
+                        // clamp, the phase within the sine unit's range
 #pragma unroll
-                    for (int j = 0; j < PX; j++) {
-                        const float s = fmaf(up, T.qx[j], sy);
-                        const float tq = fmaf(-s, s, T.q2[j]);
-                        const float g = fmaf(s, A.w, A.z);
-                        const float dl = fmaf(tq, g, -s);
-                        const float p = fmaf(dl, invbin, B.x);
-                        const int i = (int)__builtin_amdgcn_fmed3f(p, 0.f, wmax);
-                        const float4 w = L.win[I.y + i];
-                        const float vr = fmaf(p, w.z, w.x);
-                        const float vi = fmaf(p, w.w, w.y);
-                        // one revolution: v_sin/v_cos take |x| <= 256
-                        const float ph = __builtin_amdgcn_fractf(fmaf(dl, B.z, B.y));
-                        const float sn = __builtin_amdgcn_sinf(ph);
-                        const float cs = __builtin_amdgcn_cosf(ph);
-                        // z = v' + j k v
-                        const float zr = fmaf(-kk, vi, w.z * invbin);
-                        const float zi = fmaf(kk, vr, w.w * invbin);
-                        const float er = fmaf(zr, cs, -zi * sn);
-                        const float ei = fmaf(zr, sn, zi * cs);
-                        float cpn = wg[j] * fmaf(Ir[j], er, Ii[j] * ei);
-                        if (!I.w) cpn = (p > glo && p < ghi) ? cpn : 0.f;
-                        sA += cpn;
-                        sB = fmaf(cpn, dl, sB);
-                        sC = fmaf(cpn, T.qx[j], sC);
+                        for (int j = 0; j < PX; j++) {
+                            const float s = fmaf(up, T.qx[j], sy);
+                            const float tq = fmaf(-s, s, T.q2[j]);
+                            const float g = fmaf(s, A.w, A.z);
+                            const float dl = fmaf(tq, g, -s);
+                            const float p = fmaf(dl, invbin, B.x);
+                            const int i = (int)p;
+                            const float4 w = L.win[I.y + i];
+                            const float vr = fmaf(p, w.z, w.x);
+                            const float vi = fmaf(p, w.w, w.y);
+                            const float ph = fmaf(dl, B.z, B.y);
+                            const float sn = __builtin_amdgcn_sinf(ph);
+                            const float cs = __builtin_amdgcn_cosf(ph);
+                            // z / invbin = slope + j (k/invbin) v
+                            const float zr = fmaf(-kb, vi, w.z);
+                            const float zi = fmaf(kb, vr, w.w);
+                            const float er = fmaf(zr, cs, -zi * sn);
+                            const float ei = fmaf(zr, sn, zi * cs);
+                            const float cpn = fmaf(WI[j], er, WQ[j] * ei);
+                            sA += cpn;
+                            sB = fmaf(cpn, dl, sB);
+                            sC = fmaf(cpn, T.qx[j], sC);
+                        }
+                    } else {
+                        const float glo = (float)(-I.x), ghi = (float)(a.Nfft - 1 - I.x);
+#pragma unroll
+                        for (int j = 0; j < PX; j++) {
+                            const float s = fmaf(up, T.qx[j], sy);
+                            const float tq = fmaf(-s, s, T.q2[j]);
+                            const float g = fmaf(s, A.w, A.z);
+                            const float dl = fmaf(tq, g, -s);
+                            const float p = fmaf(dl, invbin, B.x);
+                            const int i = (int)__builtin_amdgcn_fmed3f(p, 0.f, wmax);
+                            const float4 w = L.win[I.y + i];
+                            const float vr = fmaf(p, w.z, w.x);
+                            const float vi = fmaf(p, w.w, w.y);
+                            // one revolution: v_sin/v_cos take |x| <= 256
+                            const float ph = __builtin_amdgcn_fractf(fmaf(dl, B.z, B.y));
+                            const float sn = __builtin_amdgcn_sinf(ph);
+                            const float cs = __builtin_amdgcn_cosf(ph);
+                            const float zr = fmaf(-kb, vi, w.z);
+                            const float zi = fmaf(kb, vr, w.w);
+                            const float er = fmaf(zr, cs, -zi * sn);
+                            const float ei = fmaf(zr, sn, zi * cs);
+                            float cpn = fmaf(WI[j], er, WQ[j] * ei);
+                            cpn = (p > glo && p < ghi) ? cpn : 0.f;
+                            sA += cpn;
+                            sB = fmaf(cpn, dl, sB);
+                            sC = fmaf(cpn, T.qx[j], sC);
+                        }
                     }
                     // sC sums over the pixel axis, sA * qy is the lane axis
                     acc[4 * k + 0] = sA;
