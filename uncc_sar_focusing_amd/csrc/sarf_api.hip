@@ -5,6 +5,7 @@
 // 93-519) with an explicit context: device buffers are caller-owned, work is
 // stream-ordered, errors are returned as codes (never printed), and the device
 // is never reset.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -196,19 +197,9 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const d
     SARF_TRY(scratch_get(ctx, "focus.img", img_elems * csz, &dimg));
     SARF_TRY(scratch_get(ctx, "focus.geo", sizeof(double) * 5 * d->Np, &dgeo));
     double *dant = (double *)dgeo, *dr0 = dant + 3 * d->Np, *df0 = dr0 + d->Np;
-    hipStream_t s = nullptr;
-    if (f64) {
-        std::vector<double> tmp(2 * ph_elems);
-        for (size_t i = 0; i < 2 * ph_elems; i++) tmp[i] = ph[i];
-        SARF_HIP(hipMemcpy(dph, tmp.data(), ph_elems * csz, hipMemcpyHostToDevice));
-    } else {
-        SARF_HIP(hipMemcpy(dph, ph, ph_elems * csz, hipMemcpyHostToDevice));
-    }
     SARF_HIP(hipMemcpy(dant, ant, sizeof(double) * 3 * d->Np, hipMemcpyHostToDevice));
     SARF_HIP(hipMemcpy(dr0, r0, sizeof(double) * d->Np, hipMemcpyHostToDevice));
     SARF_HIP(hipMemcpy(df0, f0, sizeof(double) * d->Np, hipMemcpyHostToDevice));
-    sarf_rc_desc rc{d->K, d->Np, d->Nfft, f64 ? SARF_RC_F64 : 0};
-    SARF_TRY(launch_range_compress(ctx, &rc, dph, nullptr, nullptr, nullptr, dprof, s));
     sarf_bp_desc bp{};
     bp.Nx = d->Nx; bp.Ny = d->Ny; bp.Nfft = d->Nfft; bp.Np = d->Np;
     bp.x0 = d->x0; bp.y0 = d->y0; bp.Wx = d->Wx; bp.Wy = d->Wy; bp.maxWr = d->maxWr;
@@ -230,8 +221,62 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const d
             bp.los_y = my / nrm;
         }
     }
-    SARF_TRY(launch_backproject(ctx, &bp, dprof, dant, dr0, df0, dimg, s));
-    SARF_HIP(hipStreamSynchronize(s));
+    if (f64) {
+        // validation path: whole transfers, then RC and BP
+        std::vector<double> tmp(2 * ph_elems);
+        for (size_t i = 0; i < 2 * ph_elems; i++) tmp[i] = ph[i];
+        SARF_HIP(hipMemcpy(dph, tmp.data(), ph_elems * csz, hipMemcpyHostToDevice));
+        sarf_rc_desc rc{d->K, d->Np, d->Nfft, SARF_RC_F64};
+        SARF_TRY(launch_range_compress(ctx, &rc, dph, nullptr, nullptr, nullptr, dprof, nullptr));
+        SARF_TRY(launch_backproject(ctx, &bp, dprof, dant, dr0, df0, dimg, nullptr));
+        SARF_HIP(hipStreamSynchronize(nullptr));
+    } else {
+        // Pulse groups: the upload of group g + 1 (pageable, so the copy
+        // holds this thread) overlaps range compression and backprojection
+        // of group g on the compute stream; backprojection is additive over
+        // pulses, so each group accumulates into the image.
+        const int G = std::max(1, std::min(4, d->Np / 1024));
+        hipStream_t sc = nullptr, s = nullptr;
+        SARF_HIP(hipStreamCreateWithFlags(&sc, hipStreamNonBlocking));
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipStreamDestroy(sc);
+            return fail(SARF_ERR_HIP, "sarf_focus: stream creation failed");
+        }
+        std::vector<hipEvent_t> ev(G, nullptr);
+        int rc = SARF_OK;
+        for (int g = 0; g < G && rc == SARF_OK; g++)
+            if (hipEventCreateWithFlags(&ev[g], hipEventDisableTiming) != hipSuccess)
+                rc = fail(SARF_ERR_HIP, "sarf_focus: event creation failed");
+        for (int g = 0; g < G && rc == SARF_OK; g++) {
+            const int n0 = (int)((long)d->Np * g / G), n1 = (int)((long)d->Np * (g + 1) / G);
+            const size_t off = (size_t)n0 * d->K;
+            if (hipMemcpyAsync((float2 *)dph + off, (const float2 *)ph + off,
+                               sizeof(float2) * (size_t)(n1 - n0) * d->K, hipMemcpyHostToDevice,
+                               sc) != hipSuccess ||
+                hipEventRecord(ev[g], sc) != hipSuccess || hipStreamWaitEvent(s, ev[g], 0) != hipSuccess) {
+                rc = fail(SARF_ERR_HIP, "sarf_focus: upload of pulses %d..%d failed", n0, n1);
+                break;
+            }
+            sarf_rc_desc rcd{d->K, n1 - n0, d->Nfft, 0};
+            float2 *gprof = (float2 *)dprof + (size_t)n0 * d->Nfft;
+            rc = launch_range_compress(ctx, &rcd, (float2 *)dph + off, nullptr, nullptr, nullptr,
+                                       gprof, s);
+            if (rc != SARF_OK) break;
+            sarf_bp_desc bg = bp;
+            bg.Np = n1 - n0;
+            bg.flags = g > 0 ? SARF_BP_ACCUMULATE : 0;
+            rc = launch_backproject(ctx, &bg, gprof, dant + 3 * (size_t)n0, dr0 + n0, df0 + n0,
+                                    dimg, s);
+        }
+        if (rc == SARF_OK && hipStreamSynchronize(s) != hipSuccess)
+            rc = fail(SARF_ERR_HIP, "sarf_focus: %s", hipGetErrorString(hipGetLastError()));
+        (void)hipStreamSynchronize(sc);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(s);
+        (void)hipStreamDestroy(sc);
+        if (rc != SARF_OK) return rc;
+    }
     if (f64) {
         std::vector<double> tmp(2 * img_elems);
         SARF_HIP(hipMemcpy(tmp.data(), dimg, img_elems * csz, hipMemcpyDeviceToHost));
