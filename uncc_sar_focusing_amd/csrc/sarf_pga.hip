@@ -67,6 +67,7 @@ struct PgaImage {
     double2 *ephi;         // Nx
     float *mask;           // Nx
     float2 *H;             // M: FFT of the conjugate chirp
+    float2 *wtab;          // N: chirp w_n = e^{-j pi n^2 / N}
     const float2 *twM, *twN, *ch2n;  // e^{+j 2 pi t/M}, /N, /(2N)
     int Nx, Ny, L;
 };
@@ -134,6 +135,7 @@ __device__ void setup_body(const PgaImage &p, bool spectrum) {
     for (int i = threadIdx.x; i < p.Nx; i += kThreads) p.af_ph[i] = 0.0;
     if (!spectrum) return;
     const int N = p.Nx, L = p.L, M = 1 << L;
+    for (int i = threadIdx.x; i < N; i += kThreads) p.wtab[i] = chirp(i, N, p.ch2n);
     for (int i = threadIdx.x; i < M; i += kThreads) {
         float2 v = make_float2(0.f, 0.f);
         if (i < N) v = chirp(i, N, p.ch2n);
@@ -393,7 +395,7 @@ __device__ void row_dft_lds_body(const PgaImage &p, int y) {
             if (mi >= N) mi -= N;
             const float m = p.mask[mi];
             const float2 a = row[x];
-            v = cmul(make_float2(m * a.x, m * a.y), chirp(x, N, p.ch2n));
+            v = cmul(make_float2(m * a.x, m * a.y), p.wtab[x]);
         }
         buf[x] = v;
     }
@@ -401,12 +403,12 @@ __device__ void row_dft_lds_body(const PgaImage &p, int y) {
     bluestein_core(buf, L, p.twM, p.H);
     const float invM = 1.0f / (float)M;
     const int fl = N / 2;
-    const long sh = ((long)(s - fl) % N + N) % N;
+    const int sh = ((s - fl) % N + N) % N;
     for (int k = threadIdx.x; k < N; k += kThreads) {
-        float2 X = cmul(buf[k], chirp(k, N, p.ch2n));
+        float2 X = cmul(buf[k], p.wtab[k]);
         X.x *= invM;
         X.y *= invM;
-        const float2 e = p.twN[((long)k * sh) % N];  // e^{+j 2 pi k sh / N}; use its conjugate
+        const float2 e = p.twN[((unsigned)k * (unsigned)sh) % (unsigned)N];  // e^{+j 2 pi k sh / N}; use its conjugate
         p.G[(size_t)y * N + k] = cmul(X, conjf2(e));
     }
 }
@@ -418,7 +420,7 @@ __device__ void row_apply_lds_body(const PgaImage &p, int y) {
     const int N = p.Nx, L = p.L, M = 1 << L;
     float2 *row = p.At + (size_t)y * N;
     for (int x = threadIdx.x; x < M; x += kThreads)
-        buf[x] = x < N ? cmul(row[x], chirp(x, N, p.ch2n)) : make_float2(0.f, 0.f);
+        buf[x] = x < N ? cmul(row[x], p.wtab[x]) : make_float2(0.f, 0.f);
     __syncthreads();
     bluestein_core(buf, L, p.twM, p.H);
     const float invM = 1.0f / (float)M;
@@ -427,7 +429,7 @@ __device__ void row_apply_lds_body(const PgaImage &p, int y) {
     for (int k = threadIdx.x; k < M; k += kThreads) {
         float2 v = make_float2(0.f, 0.f);
         if (k < N) {
-            const float2 w = chirp(k, N, p.ch2n);
+            const float2 w = p.wtab[k];
             const float2 X = cmul(buf[k], w);
             const double2 e = p.ephi[k];
             const float2 Z = cmul(make_float2(X.x * invM, X.y * invM),
@@ -440,7 +442,7 @@ __device__ void row_apply_lds_body(const PgaImage &p, int y) {
     bluestein_core(buf, L, p.twM, p.H);
     const float scale = invM / (float)N;
     for (int n = threadIdx.x; n < N; n += kThreads) {
-        const float2 u = cmul(buf[n], chirp(n, N, p.ch2n));
+        const float2 u = cmul(buf[n], p.wtab[n]);
         row[n] = make_float2(u.x * scale, -u.y * scale);
     }
 }
@@ -488,22 +490,22 @@ __device__ __forceinline__ void row_dft_body(const PgaImage &p, int blk) {
                 if (mi >= N) mi -= N;
                 const float m = p.mask[mi];
                 const float2 x = row[n];
-                a = cmul(make_float2(m * x.x, m * x.y), chirp(n, N, p.ch2n));
+                a = cmul(make_float2(m * x.x, m * x.y), p.wtab[n]);
             }
             v[b * S::R0 + r] = conjf2(a);
         }
     bluestein_regs<LOG2M>(v, Lr, j, p.twM, p.H);
     const float invM = 1.0f / (float)M;
     const int fl = N / 2;
-    const long sh = ((long)(s - fl) % N + N) % N;
+    const int sh = ((s - fl) % N + N) % N;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const int k = j + r * T;
         if (active && k < N) {
-            float2 X = cmul(v[r], chirp(k, N, p.ch2n));
+            float2 X = cmul(v[r], p.wtab[k]);
             X.x *= invM;
             X.y *= invM;
-            const float2 e = p.twN[((long)k * sh) % N];  // e^{+j 2 pi k sh / N}; use its conjugate
+            const float2 e = p.twN[((unsigned)k * (unsigned)sh) % (unsigned)N];  // e^{+j 2 pi k sh / N}; use its conjugate
             p.G[(size_t)y * N + k] = cmul(X, conjf2(e));
         }
     }
@@ -527,7 +529,7 @@ __device__ __forceinline__ void row_apply_body(const PgaImage &p, int blk) {
 #pragma unroll
         for (int r = 0; r < S::R0; r++) {
             const int n = j + b * T + r * (M / S::R0);
-            const float2 a = (active && n < N) ? cmul(row[n], chirp(n, N, p.ch2n))
+            const float2 a = (active && n < N) ? cmul(row[n], p.wtab[n])
                                                 : make_float2(0.f, 0.f);
             v[b * S::R0 + r] = conjf2(a);
         }
@@ -541,7 +543,7 @@ __device__ __forceinline__ void row_apply_body(const PgaImage &p, int blk) {
         const int k = j + r * T;
         float2 nx = make_float2(0.f, 0.f);
         if (k < N) {
-            const float2 w = chirp(k, N, p.ch2n);
+            const float2 w = p.wtab[k];
             const float2 X = cmul(v[r], w);
             const double2 e = p.ephi[k];
             const float2 Z = cmul(make_float2(X.x * invM, X.y * invM),
@@ -557,7 +559,7 @@ __device__ __forceinline__ void row_apply_body(const PgaImage &p, int blk) {
     for (int r = 0; r < 16; r++) {
         const int n = j + r * T;
         if (active && n < N) {
-            const float2 uo = cmul(v[r], chirp(n, N, p.ch2n));
+            const float2 uo = cmul(v[r], p.wtab[n]);
             row[n] = make_float2(uo.x * scale, -uo.y * scale);
         }
     }
@@ -695,7 +697,8 @@ size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 size_t small_bytes(int Nx, int Ny, int L) {
     return align256(sizeof(int) * Ny) + 2 * align256(sizeof(double) * kRowChunks * Nx) +
            2 * align256(sizeof(double) * Nx) + align256(sizeof(double2) * Nx) +
-           align256(sizeof(float) * Nx) + align256(sizeof(float2) * ((size_t)1 << L));
+           align256(sizeof(float) * Nx) + align256(sizeof(float2) * ((size_t)1 << L)) +
+           align256(sizeof(float2) * Nx);
 }
 
 void carve(PgaImage &p, char *base) {
@@ -712,6 +715,7 @@ void carve(PgaImage &p, char *base) {
     p.ephi = (double2 *)take(sizeof(double2) * p.Nx);
     p.mask = (float *)take(sizeof(float) * p.Nx);
     p.H = (float2 *)take(sizeof(float2) * ((size_t)1 << p.L));
+    p.wtab = (float2 *)take(sizeof(float2) * p.Nx);
 }
 
 int tables(sarf_ctx *ctx, PgaImage &p) {
