@@ -551,7 +551,9 @@ def serving_leg(ctx, torch, n_scenes: int = 64, pga_iters: int = 5, n_streams: i
     """64 uneven scenes (pulses 256-4096, samples 256-2048, sides 128-1024)
     with an 8th-order azimuth phase error, each range-compressed,
     backprojected and corrected by 5 PGA iterations.  Batched: all scenes'
-    backprojection in one persistent launch, then all scenes' PGA in one
+    range compression in one sarf_range_compress_scenes call (one launch
+    per transform length), their backprojection in one persistent launch,
+    then all scenes' PGA in one
     sarf_pga_scenes call (each PGA step one launch over all images); also
     timed: the same batch with per-image PGA calls spread over `n_streams`
     streams (one context each); latency: each scene alone on one stream."""
@@ -571,22 +573,25 @@ def serving_leg(ctx, torch, n_scenes: int = 64, pga_iters: int = 5, n_streams: i
     descs = [(profs[i], s.ant, s.r0, s.f0, s.grid, s.geom.maxWr, imgs[i],
               s.geom.los(s.grid.x0, s.grid.y0)) for i, s in enumerate(scenes)]
 
+    phs = [s.ph for s in scenes]
+    nffts = [s.nfft for s in scenes]
     # batch tile for the one-launch backprojection of all scenes
-    for s, p in zip(scenes, profs):
-        ctx.range_compress(s.ph, s.nfft, out=p)
+    ctx.range_compress_scenes(phs, nffts, out=profs)
     scene_tile, _ = ctx.autotune_scenes(descs)
 
-    def rc_bp():
-        for s, p in zip(scenes, profs):
-            ctx.range_compress(s.ph, s.nfft, out=p)
+    def rc_bp(e_rc=None):
+        ctx.range_compress_scenes(phs, nffts, out=profs)
+        if e_rc is not None:
+            e_rc.record()
         ctx.backproject_scenes(descs, tile=scene_tile)
 
-    def batch():
-        rc_bp()
+    def batch(timed=False):
+        e_rc = torch.cuda.Event(enable_timing=True) if timed else None
+        rc_bp(e_rc)
         e_bp = torch.cuda.Event(enable_timing=True)
         e_bp.record()
         ctx.pga_scenes(imgs, pga_iters, out=pga_out)
-        return e_bp
+        return e_rc, e_bp
 
     order = sorted(range(len(imgs)), key=lambda i: -imgs[i].numel())  # big scenes first
 
@@ -611,11 +616,12 @@ def serving_leg(ctx, torch, n_scenes: int = 64, pga_iters: int = 5, n_streams: i
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
-    e_bp = batch()
+    e_rc, e_bp = batch(timed=True)
     e1.record()
     e1.synchronize()
     t_batch = e0.elapsed_time(e1)
     t_rcbp = e0.elapsed_time(e_bp)
+    t_rc = e0.elapsed_time(e_rc)
     e0.record()
     batch_streams()
     e1.record()
@@ -651,8 +657,7 @@ def serving_leg(ctx, torch, n_scenes: int = 64, pga_iters: int = 5, n_streams: i
         log(f"bench: serving graph capture failed: {e}")
     # precision of the batched backprojection (before PGA) against the
     # all-FP64 path, every scene
-    for s, p in zip(scenes, profs):
-        ctx.range_compress(s.ph, s.nfft, out=p)
+    ctx.range_compress_scenes(phs, nffts, out=profs)
     ctx.backproject_scenes(descs, tile=scene_tile)
     rels = []
     for i, s in enumerate(scenes):
@@ -678,6 +683,8 @@ def serving_leg(ctx, torch, n_scenes: int = 64, pga_iters: int = 5, n_streams: i
         "pixel_pulses": int(total_pp),
         "batch_gpu_ms": round(t_batch, 3),
         "batch_rc_bp_gpu_ms": round(t_rcbp, 3),
+        "batch_rc_gpu_ms": round(t_rc, 3),
+        "rc": "batched (sarf_range_compress_scenes)",
         "batch_Gpixel_pulses_per_s": round(total_pp / (t_batch * 1e-3) / 1e9, 1),
         "batch_rc_bp_Gpixel_pulses_per_s": round(total_pp / (t_rcbp * 1e-3) / 1e9, 1),
         "scenes_tile": scene_tile,

@@ -94,6 +94,29 @@ int sarf_range_compress(sarf_ctx *ctx, const sarf_rc_desc *desc,
                         void *prof,           /* device, complex Nfft x Np */
                         void *stream);
 
+/* Range compression of many independent scenes (FP32), each as one         */
+/* sarf_range_compress call: the scenes of one Nfft share one persistent    */
+/* launch over all their pulses.  Results equal per-scene calls bit for bit.*/
+/* The scene list is uploaded (and the call waits for the copy) only when   */
+/* it differs from the previous call on this context, so a repeated call    */
+/* can be captured into a HIP graph (a new list during capture returns      */
+/* SARF_ERR_INVALID).  The reference compresses one aperture per call       */
+/* (cuda_sar_focusing.hpp:616-621).                                         */
+typedef struct {
+    const void *ph;       /* device float2 K x Np */
+    void *prof;           /* device float2 Nfft x Np */
+    const float *taper;   /* device, K (SARF_RC_TAPER) or NULL */
+    const double *f0;     /* device, Np (SARF_RC_RAMP) or NULL */
+    const double *r0;     /* device, Np (SARF_RC_RAMP) or NULL */
+    int K, Np;            /* as in sarf_rc_desc */
+    int Nfft;             /* power of two, 32 <= Nfft <= 16384 */
+    int flags;            /* SARF_RC_TAPER | SARF_RC_RAMP */
+} sarf_rc_scene;
+
+int sarf_range_compress_scenes(sarf_ctx *ctx, int nscenes,
+                               const sarf_rc_scene *scenes, /* host array */
+                               void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Backprojection of range profiles onto the z = 0 pixel grid.              */
 /* Replaces backprojection_loop (cuda_BackProjectionKernels.cuh:33-93) and  */

@@ -89,6 +89,70 @@ def test_rc_rejects_bad_sizes(ctx):
     assert e.value.code == _lib.SARF_ERR_UNSUPPORTED
 
 
+def _rc_scene_inputs(seed):
+    """Uneven scenes spanning every transform-length class the batched call
+    groups by: several per Nfft, 4 and 16 sample rows per lane (K <= Nfft/4
+    or not), one pulse and pulse counts that leave partial pulse groups,
+    taper and ramp on some."""
+    rng = np.random.default_rng(seed)
+    specs = [(1, 20, 32), (7, 32, 32), (33, 5, 64), (9, 100, 256), (300, 64, 256),
+             (17, 1000, 1024), (5, 200, 1024), (64, 256, 4096), (3, 4096, 4096),
+             (40, 1500, 8192), (2, 300, 16384), (11, 9000, 16384)]
+    out = []
+    for i, (Np, K, nfft) in enumerate(specs):
+        ph = (rng.standard_normal((Np, K)) + 1j * rng.standard_normal((Np, K))).astype(np.complex64)
+        taper = np.hanning(K).astype(np.float32) if i % 3 == 1 else None
+        ramp = (9.6e9 + 1e6 * np.arange(Np), 12500.0 + 0.37 * np.arange(Np)) if i % 4 == 2 \
+            else None
+        out.append((ph, nfft, taper, ramp))
+    return out
+
+
+def test_rc_scenes_equal_per_scene_calls(ctx):
+    """sarf_range_compress_scenes: bit-identical to one sarf_range_compress
+    call per scene (same kernel body), and within 2e-6 rel-L2 of the FP64
+    oracle; a second call with the same list (cached upload) and a call
+    with fewer scenes give the same results."""
+    inp = _rc_scene_inputs(11)
+    phs = [to_dev(ph) for ph, _, _, _ in inp]
+    tapers = [None if t is None else to_dev(t) for _, _, t, _ in inp]
+    ramps = [None if r is None else (to_dev(r[0]), to_dev(r[1])) for _, _, _, r in inp]
+    nffts = [n for _, n, _, _ in inp]
+    single = [ctx.range_compress(p, n, taper=t, ramp=r).cpu()
+              for p, n, t, r in zip(phs, nffts, tapers, ramps)]
+    outs = ctx.range_compress_scenes(phs, nffts, tapers=tapers, ramps=ramps)
+    for i, (o, s1) in enumerate(zip(outs, single)):
+        assert torch.equal(o.cpu(), s1), i
+        ph, nfft, taper, ramp = inp[i]
+        ref = oracle.range_compress_f64(ph, nfft, taper=taper, ramp=ramp)
+        assert oracle.rel_l2(o.cpu().numpy(), ref) < 2e-6, i
+    for o in outs:
+        o.zero_()
+    ctx.range_compress_scenes(phs, nffts, tapers=tapers, ramps=ramps, out=outs)
+    for i, (o, s1) in enumerate(zip(outs, single)):
+        assert torch.equal(o.cpu(), s1), i
+    sub = [1, 4, 9]
+    outs2 = ctx.range_compress_scenes([phs[i] for i in sub], [nffts[i] for i in sub],
+                                      tapers=[tapers[i] for i in sub],
+                                      ramps=[ramps[i] for i in sub])
+    for o, i in zip(outs2, sub):
+        assert torch.equal(o.cpu(), single[i]), i
+
+
+def test_rc_scenes_rejects_bad_scenes(ctx):
+    ph = to_dev(np.zeros((2, 10), np.complex64))
+    ok = to_dev(np.zeros((2, 16), np.complex64))
+    with pytest.raises(_lib.SarfError) as e:
+        ctx.range_compress_scenes([ok, ph], [32, 8])  # K > Nfft in scene 1
+    assert e.value.code == _lib.SARF_ERR_INVALID and "scene 1" in str(e.value)
+    with pytest.raises(_lib.SarfError) as e:
+        ctx.range_compress_scenes([ph], [24])  # not a power of two
+    assert e.value.code == _lib.SARF_ERR_UNSUPPORTED
+    with pytest.raises(_lib.SarfError) as e:
+        ctx.range_compress_scenes([], [])
+    assert e.value.code == _lib.SARF_ERR_INVALID
+
+
 # ----------------------------------------------------------- backprojection
 def test_bp_c0_full_matches_fp64_oracle(ctx):
     """Configuration C0 in full: RC + mixed BP on the GPU vs FP64 oracle.

@@ -1,5 +1,7 @@
 """HIP graph capture of a serving batch: range compression of two uneven
-scenes, their backprojection in one sarf_bp_scenes launch and PGA, captured
+scenes (per scene, or batched in one sarf_range_compress_scenes call), their
+backprojection in one sarf_bp_scenes launch and PGA (per image, or batched
+in one sarf_pga_scenes call), captured
 once (after one eager call that uploads the scene list) and replayed
 several times with other GPU work and synchronisations in between.  Every
 replay must reproduce the eager result exactly (the kernels are
@@ -19,8 +21,8 @@ def _dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
 
 
-@pytest.mark.parametrize("pga_mode", ["per_image", "batched"])
-def test_serving_batch_graph_replays_match_eager(ctx, pga_mode):
+@pytest.mark.parametrize("mode", ["per_scene", "batched"])
+def test_serving_batch_graph_replays_match_eager(ctx, mode):
     specs = [(48, 96, ImageGrid(70, 45, 0.3, -0.2, 6.9, 4.4), 1024),
              (31, 64, ImageGrid(40, 52, -0.1, 0.2, 3.9, 5.1), 512)]
     scenes = []
@@ -38,10 +40,14 @@ def test_serving_batch_graph_replays_match_eager(ctx, pga_mode):
              torch.empty(2, dtype=torch.float64, device=DEV)) for s in scenes]
 
     def batch():
-        for s in scenes:
-            ctx.range_compress(s["ph"], s["nfft"], out=s["prof"])
+        if mode == "batched":
+            ctx.range_compress_scenes([s["ph"] for s in scenes], [s["nfft"] for s in scenes],
+                                      out=[s["prof"] for s in scenes])
+        else:
+            for s in scenes:
+                ctx.range_compress(s["ph"], s["nfft"], out=s["prof"])
         ctx.backproject_scenes(descs)
-        if pga_mode == "batched":
+        if mode == "batched":
             ctx.pga_scenes([s["img"] for s in scenes], 2, out=outs)
         else:
             for s in scenes:

@@ -67,6 +67,7 @@ SIGNATURES = {
     "sarf_ctx_create": (_i, [_i, C.POINTER(_vp)]),
     "sarf_ctx_destroy": (_i, [_vp]),
     "sarf_range_compress": (_i, [_vp, C.POINTER(RcDesc), _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sarf_range_compress_scenes": (_i, [_vp, _i, C.c_void_p, _vp]),
     "sarf_backproject": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _vp, _vp]),
     "sarf_focus": (_i, [_vp, C.POINTER(FocusDesc), _vp, _vp, _vp, _vp, _vp]),
     "sarf_synth_phase_history": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp]),
@@ -110,6 +111,11 @@ class SceneDesc(C.Structure):
 
 class PgaDesc(C.Structure):
     _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("iters", C.c_int), ("db_down", C.c_double)]
+
+
+class RcScene(C.Structure):
+    _fields_ = [("ph", _vp), ("prof", _vp), ("taper", _vp), ("f0", _vp), ("r0", _vp),
+                ("K", C.c_int), ("Np", C.c_int), ("Nfft", C.c_int), ("flags", C.c_int)]
 
 
 class PgaScene(C.Structure):

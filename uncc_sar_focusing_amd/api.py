@@ -90,6 +90,42 @@ class Context:
                                            _ptr(r0), _ptr(out), self._stream()))
         return out
 
+    def range_compress_scenes(self, phs, nffts, *, tapers=None, ramps=None,
+                              out=None) -> list:
+        """range_compress of many complex64 phase histories (shapes and nfft
+        may differ) in one call, one launch per transform length
+        (sarf_range_compress_scenes).  `tapers` / `ramps` optionally give a
+        per-scene taper tensor / (f0, r0) pair or None; `out` may pass the
+        profile tensors to write (for graph capture)."""
+        n = len(phs)
+        if len(nffts) != n:
+            raise ValueError("range_compress_scenes: one nfft per phase history")
+        descs = (_lib.RcScene * n)()
+        res = []
+        for i, ph in enumerate(phs):
+            Np, K = ph.shape
+            nfft = int(nffts[i])
+            _need(ph, torch.complex64, (Np, K), f"scene {i} ph")
+            flags = 0
+            taper = tapers[i] if tapers is not None else None
+            if taper is not None:
+                _need(taper, torch.float32, (K,), f"scene {i} taper")
+                flags |= _lib.SARF_RC_TAPER
+            f0 = r0 = None
+            if ramps is not None and ramps[i] is not None:
+                f0, r0 = ramps[i]
+                _need(f0, torch.float64, (Np,), f"scene {i} f0")
+                _need(r0, torch.float64, (Np,), f"scene {i} r0")
+                flags |= _lib.SARF_RC_RAMP
+            prof = out[i] if out is not None else torch.empty(
+                (Np, nfft), dtype=torch.complex64, device=ph.device)
+            _need(prof, torch.complex64, (Np, nfft), f"scene {i} out")
+            descs[i] = _lib.RcScene(*(0 if t is None else t.data_ptr()
+                                      for t in (ph, prof, taper, f0, r0)), K, Np, nfft, flags)
+            res.append(prof)
+        check(self.lib.sarf_range_compress_scenes(self._h, n, descs, self._stream()))
+        return res
+
     # ------------------------------------------------------------------ BP
     def backproject(self, prof: torch.Tensor, ant: torch.Tensor, r0: torch.Tensor,
                     f0: torch.Tensor, grid: ImageGrid, maxWr: float, *,

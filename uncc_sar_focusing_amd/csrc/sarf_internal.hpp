@@ -62,6 +62,9 @@ struct sarf_ctx {
     // tile), so an identical call re-launches without a copy or a wait
     std::vector<unsigned char> scenes_key;
     void *scenes_buf = nullptr;
+    // last scene list uploaded by sarf_range_compress_scenes
+    std::vector<unsigned char> rcs_key;
+    void *rcs_buf = nullptr;
     // last image list uploaded by sarf_pga_scenes (device state bytes)
     std::vector<unsigned char> pga_key;
     void *pga_buf = nullptr;
