@@ -65,6 +65,7 @@ constexpr int bp_waves_per_eu() {
 }
 constexpr int kBpChunk = 32;  // max pulses staged per chunk
 constexpr int kBpCap = 2048;  // LDS window capacity, in bins (16 B each)
+static_assert(kBpChunk <= 64 && kBpCap % kBpThreads == 0, "staging layout");
 
 struct BpArgs {
     int Nx, Ny, Nfft, Np;
@@ -84,7 +85,8 @@ struct BpLds {
     int4 pI[kBpChunk];    // start bin, LDS offset (bins), W, interior
     float4 pC[kBpChunk];  // separable terms (lane axis): c2l, c3l, e1, e2
     float4 pD[kBpChunk];  // separable terms: e3, c2p, c3p (pixel axis), -
-    int nc;
+    int nc;   // pulses staged
+    int tot;  // window bins staged
 };
 
 // Antenna positions as given.
@@ -180,6 +182,8 @@ struct BpTile {
                                           float *fx = nullptr) {
         const int tid = threadIdx.x;
         __syncthreads();  // previous users of L are done
+        // Wave 0, one lane per pulse of the chunk (cmax <= kBpChunk <= 64).
+        int W = 0, start = 0, interior = 0;
         if (tid < cmax) {
             const int n = n0 + tid;
             const double3 an = pos(n);
@@ -191,16 +195,17 @@ struct BpTile {
             // |dR - dR_c| over the tile <= |uh_x| hx + |uh_y| hy + |q|^2/(2 R_c)
             const double ext = fabs(ux * iR) * hx + fabs(uy * iR) * hy + (hx * hx + hy * hy) * 0.5 * iR;
             const double span = ext * a.Nfft / a.maxWr;
-            const long start = (long)floor(posg - span) - 1;
+            const long st = (long)floor(posg - span) - 1;
             // Clamped so a pulse always fits; the host checks the worst case so
             // the clamp does not bind for valid geometry.
-            const int W = min((int)((long)ceil(posg + span) - start) + 2, kBpCap);
+            W = min((int)((long)ceil(posg + span) - st) + 2, kBpCap);
+            start = (int)st;
             const double kap = 2.0 * f0[n] / kClight;  // revolutions per metre
             double psi = kap * dRc;
             psi -= rint(psi);
             L.pA[tid] = make_float4((float)(ux * iR), (float)(uy * iR), (float)(0.5 * iR),
                                     (float)(0.5 * iR * iR));
-            L.pB[tid] = make_float4((float)(posg - (double)start), (float)psi, (float)kap,
+            L.pB[tid] = make_float4((float)(posg - (double)st), (float)psi, (float)kap,
                                     (float)(W - 2));
             {
                 // separable form of the same third-order expansion (u_p, u_l:
@@ -221,34 +226,64 @@ struct BpTile {
             // Fast path: the window lies inside the profile (no swath gate) and
             // |psi| <= 0.5 + kap * ext stays inside v_sin/v_cos's input range
             // (|x| <= 256 revolutions), so the phase needs no reduction.
-            const int interior = (start >= 0 && start + W <= a.Nfft && kap * ext <= 250.0) ? 1 : 0;
-            L.pI[tid] = make_int4((int)start, 0, W, interior);
+            interior = (st >= 0 && st + W <= a.Nfft && kap * ext <= 250.0) ? 1 : 0;
         }
-        __syncthreads();
-        if (tid == 0) {
-            int off = 0, c = 0;
-            for (; c < cmax; c++) {
-                const int W = L.pI[c].z;
-                if (off + W > kBpCap) break;
-                L.pI[c].y = off;
-                off += W;
+        if (tid < 64) {
+            // Windows are packed back to back: offsets are the exclusive
+            // prefix sum of W over the chunk (a wave scan), and the pulses
+            // whose windows fit in LDS are a prefix of the chunk.
+            int incl = W;
+#pragma unroll
+            for (int d = 1; d < kBpChunk; d <<= 1) {
+                const int t = __shfl_up(incl, d, 64);
+                if (tid >= d) incl += t;
             }
-            L.nc = c;
+            const int nc = __popcll(__ballot(tid < cmax && incl <= kBpCap));
+            if (tid < cmax) L.pI[tid] = make_int4(start, incl - W, W, interior);
+            if (tid == nc - 1) {
+                L.nc = nc;
+                L.tot = incl;
+            }
         }
         __syncthreads();
-        const int nc = L.nc;
-        const int wave = tid >> 6, lane = tid & 63;
-        for (int c = wave; c < nc; c += kBpThreads / 64) {
-            const int4 I = L.pI[c];
-            const float2 *src = prof + (size_t)(n0 + c) * a.Nfft;
-            for (int b = lane; b < I.z; b += 64) {
-                const long g = (long)I.x + b;
-                float2 s0 = make_float2(0.f, 0.f), s1 = make_float2(0.f, 0.f);
-                if (g >= 0 && g < a.Nfft) s0 = src[g];
-                if (g + 1 >= 0 && g + 1 < a.Nfft) s1 = src[g + 1];
-                const float dr = s1.x - s0.x, di = s1.y - s0.y;
-                const float fb = (float)b;
-                L.win[I.y + b] = make_float4(fmaf(-fb, dr, s0.x), fmaf(-fb, di, s0.y), dr, di);
+        const int nc = L.nc, tot = L.tot;
+        // Window fill over the chunk's bins as one flat range, in rounds of
+        // kSlots bins per thread whose loads are all issued before any is
+        // consumed (one memory latency per round instead of one per pulse):
+        // bin f belongs to the last pulse c with offset <= f.
+        constexpr int kSlots = 4;
+        const float2 *__restrict__ base = prof + (size_t)n0 * a.Nfft;  // chunk's first profile
+        for (int f0r = 0; f0r < tot; f0r += kSlots * kBpThreads) {
+            float2 v0[kSlots], v1[kSlots];
+            int wb[kSlots];
+#pragma unroll
+            for (int k = 0; k < kSlots; k++) {
+                const int f = f0r + tid + k * kBpThreads;
+                v0[k] = make_float2(0.f, 0.f);
+                v1[k] = make_float2(0.f, 0.f);
+                wb[k] = 0;
+                if (f < tot) {
+                    int c = 0;
+#pragma unroll
+                    for (int step = kBpChunk / 2; step >= 1; step >>= 1)
+                        if (c + step < nc && L.pI[c + step].y <= f) c += step;
+                    const int4 I = L.pI[c];
+                    const int b = f - I.y;
+                    const int g = I.x + b;  // profile bin; |g| < 2 Nfft
+                    const unsigned o = (unsigned)(c * a.Nfft + g);  // < kBpChunk * Nfft
+                    if (g >= 0 && g < a.Nfft) v0[k] = base[o];
+                    if (g + 1 >= 0 && g + 1 < a.Nfft) v1[k] = base[o + 1];
+                    wb[k] = b;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kSlots; k++) {
+                const int f = f0r + tid + k * kBpThreads;
+                if (f < tot) {
+                    const float dr = v1[k].x - v0[k].x, di = v1[k].y - v0[k].y;
+                    const float fb = (float)wb[k];
+                    L.win[f] = make_float4(fmaf(-fb, dr, v0[k].x), fmaf(-fb, di, v0[k].y), dr, di);
+                }
             }
         }
         if (kSep && fx != nullptr) {
