@@ -6,9 +6,12 @@
 // stream-ordered, errors are returned as codes (never printed), and the device
 // is never reset.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "sarf_internal.hpp"
@@ -114,6 +117,113 @@ int twiddles_f64(sarf_ctx *ctx, int log2n, const double2 **out) {
     return twiddles_any_f64(ctx, 1L << log2n, out);
 }
 
+// Copies between the caller's pageable buffers and pinned staging.  One
+// thread copying pageable memory moves a few GB/s; the copy is split into
+// parts taken by the calling thread and a few persistent workers.  A job is
+// a generation number; a part is claimed by a compare-and-swap on
+// (generation << 32 | part), so a worker still looking at an older job can
+// never take a part of a newer one.
+class HostCopier {
+  public:
+    explicit HostCopier(int nthreads) {
+        for (int i = 0; i + 1 < nthreads; i++) th_.emplace_back([this] { work(); });
+    }
+    ~HostCopier() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (std::thread &t : th_) t.join();
+    }
+    // Blocking parallel memcpy of n bytes.
+    void copy(void *dst, const void *src, size_t n) {
+        constexpr size_t kPart = size_t(1) << 20;
+        if (th_.empty() || n <= 2 * kPart) {
+            std::memcpy(dst, src, n);
+            return;
+        }
+        Job j{(char *)dst, (const char *)src, n, (int)std::min<size_t>(256, (n + kPart - 1) / kPart)};
+        uint32_t g;
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            g = ++gen_;
+            job_ = j;
+            left_.store(j.parts);
+            ticket_.store((uint64_t)g << 32);
+        }
+        cv_.notify_all();
+        chew(j, g);
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [&] { return left_.load() == 0; });
+    }
+
+  private:
+    struct Job {
+        char *dst;
+        const char *src;
+        size_t n;
+        int parts;
+    };
+    void chew(const Job &j, uint32_t g) {
+        for (;;) {
+            uint64_t t = ticket_.load();
+            int p;
+            do {
+                if ((uint32_t)(t >> 32) != g) return;  // a newer job
+                p = (int)(uint32_t)t;
+                if (p >= j.parts) return;
+            } while (!ticket_.compare_exchange_weak(t, t + 1));
+            const size_t b = j.n * p / j.parts, e = j.n * (p + 1) / j.parts;
+            std::memcpy(j.dst + b, j.src + b, e - b);
+            if (left_.fetch_sub(1) == 1) {
+                std::lock_guard<std::mutex> lk(m_);
+                done_.notify_all();
+            }
+        }
+    }
+    void work() {
+        uint32_t seen = 0;
+        for (;;) {
+            Job j;
+            uint32_t g;
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = g = gen_;
+                j = job_;
+            }
+            chew(j, g);
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    Job job_{};
+    uint32_t gen_ = 0;
+    bool stop_ = false;
+    std::atomic<uint64_t> ticket_{0};
+    std::atomic<int> left_{0};
+};
+
+// Pinned staging of at least `bytes` bytes for the host-buffer path.
+int pinned_get(sarf_ctx *ctx, size_t bytes, char **out) {
+    if (ctx->pinned_bytes < bytes) {
+        if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+        ctx->pinned = nullptr;
+        ctx->pinned_bytes = 0;
+        SARF_HIP(hipHostMalloc(&ctx->pinned, bytes, hipHostMallocDefault));
+        ctx->pinned_bytes = bytes;
+    }
+    if (!ctx->copier) {
+        const unsigned hw = std::thread::hardware_concurrency();
+        ctx->copier = new HostCopier((int)std::max(1u, std::min(8u, hw ? hw : 1u)));
+    }
+    *out = (char *)ctx->pinned;
+    return SARF_OK;
+}
+
 }  // namespace sarf
 
 using namespace sarf;
@@ -155,6 +265,8 @@ int sarf_ctx_destroy(sarf_ctx *ctx) {
     for (auto &kv : ctx->tw64) (void)hipFree(kv.second);
     for (auto &kv : ctx->scratch)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    delete ctx->copier;
     delete ctx;
     return SARF_OK;
 }
@@ -231,29 +343,52 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const d
         SARF_TRY(launch_backproject(ctx, &bp, dprof, dant, dr0, df0, dimg, nullptr));
         SARF_HIP(hipStreamSynchronize(nullptr));
     } else {
-        // Pulse groups: the upload of group g + 1 (pageable, so the copy
-        // holds this thread) overlaps range compression and backprojection
-        // of group g on the compute stream; backprojection is additive over
-        // pulses, so each group accumulates into the image.
+        // Pulse groups: the caller's pageable samples of group g + 1 are
+        // copied into pinned staging (several threads) and sent to the GPU
+        // while group g is range-compressed and backprojected; the image comes
+        // back in chunks, each copied out of pinned staging while the next is
+        // in flight.  Backprojection is additive over pulses, so each group
+        // accumulates into the image.
         const int G = std::max(1, std::min(4, d->Np / 1024));
+        size_t gmax = 0;
+        for (int g = 0; g < G; g++) {
+            const int n0 = (int)((long)d->Np * g / G), n1 = (int)((long)d->Np * (g + 1) / G);
+            gmax = std::max(gmax, sizeof(float2) * (size_t)(n1 - n0) * d->K);
+        }
+        const size_t slot = (gmax + 255) & ~(size_t)255, img_bytes = img_elems * csz;
+        char *pin = nullptr;
+        SARF_TRY(pinned_get(ctx, 2 * slot + img_bytes, &pin));
+        char *pin_ph[2] = {pin, pin + slot};
+        char *pin_img = pin + 2 * slot;
+        constexpr int kChunks = 4;
         hipStream_t sc = nullptr, s = nullptr;
         SARF_HIP(hipStreamCreateWithFlags(&sc, hipStreamNonBlocking));
         if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
             (void)hipStreamDestroy(sc);
             return fail(SARF_ERR_HIP, "sarf_focus: stream creation failed");
         }
-        std::vector<hipEvent_t> ev(G, nullptr);
+        hipEvent_t copied[2] = {nullptr, nullptr}, dl[kChunks] = {};
         int rc = SARF_OK;
-        for (int g = 0; g < G && rc == SARF_OK; g++)
-            if (hipEventCreateWithFlags(&ev[g], hipEventDisableTiming) != hipSuccess)
+        for (hipEvent_t *e : {&copied[0], &copied[1]})
+            if (rc == SARF_OK && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess)
+                rc = fail(SARF_ERR_HIP, "sarf_focus: event creation failed");
+        for (int c = 0; c < kChunks && rc == SARF_OK; c++)
+            if (hipEventCreateWithFlags(&dl[c], hipEventDisableTiming) != hipSuccess)
                 rc = fail(SARF_ERR_HIP, "sarf_focus: event creation failed");
         for (int g = 0; g < G && rc == SARF_OK; g++) {
             const int n0 = (int)((long)d->Np * g / G), n1 = (int)((long)d->Np * (g + 1) / G);
-            const size_t off = (size_t)n0 * d->K;
-            if (hipMemcpyAsync((float2 *)dph + off, (const float2 *)ph + off,
-                               sizeof(float2) * (size_t)(n1 - n0) * d->K, hipMemcpyHostToDevice,
-                               sc) != hipSuccess ||
-                hipEventRecord(ev[g], sc) != hipSuccess || hipStreamWaitEvent(s, ev[g], 0) != hipSuccess) {
+            const size_t off = (size_t)n0 * d->K, bytes = sizeof(float2) * (size_t)(n1 - n0) * d->K;
+            const int k = g & 1;
+            // staging slot k was last sent for group g - 2
+            if (g >= 2 && hipEventSynchronize(copied[k]) != hipSuccess) {
+                rc = fail(SARF_ERR_HIP, "sarf_focus: upload of group %d failed", g - 2);
+                break;
+            }
+            ctx->copier->copy(pin_ph[k], (const float2 *)ph + off, bytes);
+            if (hipMemcpyAsync((float2 *)dph + off, pin_ph[k], bytes, hipMemcpyHostToDevice, sc) !=
+                    hipSuccess ||
+                hipEventRecord(copied[k], sc) != hipSuccess ||
+                hipStreamWaitEvent(s, copied[k], 0) != hipSuccess) {
                 rc = fail(SARF_ERR_HIP, "sarf_focus: upload of pulses %d..%d failed", n0, n1);
                 break;
             }
@@ -268,10 +403,26 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const d
             rc = launch_backproject(ctx, &bg, gprof, dant + 3 * (size_t)n0, dr0 + n0, df0 + n0,
                                     dimg, s);
         }
-        if (rc == SARF_OK && hipStreamSynchronize(s) != hipSuccess)
-            rc = fail(SARF_ERR_HIP, "sarf_focus: %s", hipGetErrorString(hipGetLastError()));
+        for (int c = 0; c < kChunks && rc == SARF_OK; c++) {
+            const size_t b = img_bytes * c / kChunks, e = img_bytes * (c + 1) / kChunks;
+            if (hipMemcpyAsync(pin_img + b, (char *)dimg + b, e - b, hipMemcpyDeviceToHost, s) !=
+                    hipSuccess ||
+                hipEventRecord(dl[c], s) != hipSuccess)
+                rc = fail(SARF_ERR_HIP, "sarf_focus: image download failed");
+        }
+        for (int c = 0; c < kChunks && rc == SARF_OK; c++) {
+            const size_t b = img_bytes * c / kChunks, e = img_bytes * (c + 1) / kChunks;
+            if (hipEventSynchronize(dl[c]) != hipSuccess) {
+                rc = fail(SARF_ERR_HIP, "sarf_focus: %s", hipGetErrorString(hipGetLastError()));
+                break;
+            }
+            ctx->copier->copy((char *)img + b, pin_img + b, e - b);
+        }
+        (void)hipStreamSynchronize(s);
         (void)hipStreamSynchronize(sc);
-        for (hipEvent_t e : ev)
+        for (hipEvent_t e : copied)
+            if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : dl)
             if (e) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(s);
         (void)hipStreamDestroy(sc);
@@ -281,8 +432,6 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const d
         std::vector<double> tmp(2 * img_elems);
         SARF_HIP(hipMemcpy(tmp.data(), dimg, img_elems * csz, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < 2 * img_elems; i++) img[i] = (float)tmp[i];
-    } else {
-        SARF_HIP(hipMemcpy(img, dimg, img_elems * csz, hipMemcpyDeviceToHost));
     }
     return SARF_OK;
 }

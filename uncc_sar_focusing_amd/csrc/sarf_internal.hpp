@@ -44,6 +44,10 @@ struct Scratch {
     size_t bytes = 0;
 };
 
+// Parallel memcpy between pageable and pinned host memory on a few
+// persistent threads (defined with sarf_focus).
+class HostCopier;
+
 }  // namespace sarf
 
 struct sarf_ctx {
@@ -65,6 +69,10 @@ struct sarf_ctx {
     // last scene list uploaded by sarf_range_compress_scenes
     std::vector<unsigned char> rcs_key;
     void *rcs_buf = nullptr;
+    // host-buffer path: pinned staging (grown on demand) and its copy threads
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+    sarf::HostCopier *copier = nullptr;
     // last image list uploaded by sarf_pga_scenes (device state bytes)
     std::vector<unsigned char> pga_key;
     void *pga_buf = nullptr;
