@@ -207,6 +207,27 @@ int sarf_autotune(sarf_ctx *ctx, const sarf_bp_desc *desc, const void *prof, con
                   const double *r0, const double *f0, void *img, int reps,
                   sarf_tune_result *out, void *stream);
 
+/* The launch shapes sarf_autotune remembered, as a table a caller can save */
+/* (e.g. as JSON) and load into the context of a later process, so that a   */
+/* serving process does not tune again.  An entry holds the geometry it     */
+/* applies to -- sizes, spacings W/(N-1), maxWr and the LOS hint, matched   */
+/* exactly -- and the chosen tile and pulse split.  No reference            */
+/* counterpart (the reference has no tuner).                                */
+typedef struct {
+    int Nx, Ny, Np, Nfft;
+    double dx, dy;        /* Wx/(Nx-1), Wy/(Ny-1) as sarf_bp_desc gives them */
+    double maxWr;
+    double los_x, los_y;  /* the LOS hint of the tuned descriptor (0, 0 = none) */
+    int tile, pulse_split;
+} sarf_tune_entry;
+
+/* *count = number of remembered shapes; the first min(cap, *count) are     */
+/* copied to out (out may be NULL when cap = 0).                            */
+int sarf_tune_table_get(sarf_ctx *ctx, sarf_tune_entry *out, int cap, int *count);
+/* Adds or replaces remembered shapes; every entry is validated (known tile, */
+/* 1 <= pulse_split <= Np, positive spacings) before any is stored.         */
+int sarf_tune_table_set(sarf_ctx *ctx, const sarf_tune_entry *entries, int n);
+
 /* ------------------------------------------------------------------------ */
 /* Image entropy E = -sum p ln p, p = |I|^2 / sum |I|^2 (FP64 sums).         */
 /* out (device, 3 doubles) = {E, S = sum |I|^2, T = sum |I|^2 ln |I|^2}.     */

@@ -423,6 +423,50 @@ def test_autotune_chooses_an_accurate_shape_and_is_used_by_default():
         tctx.close()
 
 
+def test_tune_table_round_trips_through_json_into_another_context():
+    """The remembered shapes export as a JSON-ready table; loaded into a new
+    context they steer its default-shaped calls on the same geometry (here
+    a pair other than the default, so the effect is visible: the default
+    call equals that explicit pair bit for bit and not the untuned default),
+    and a table with an unknown tile is refused whole."""
+    import json
+
+    from uncc_sar_focusing_amd.api import Context
+
+    cfg, geom, tg, grid, ph = c0_inputs()
+    a_ctx, b_ctx = Context(0), Context(0)
+    try:
+        ant, r0, f0 = geom_dev(geom)
+        prof = a_ctx.range_compress(to_dev(ph), cfg.nfft)
+        untuned = b_ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr).cpu()
+        res, _ = a_ctx.autotune(prof, ant, r0, f0, grid, geom.maxWr, reps=1)
+        table = json.loads(json.dumps(a_ctx.tune_table()))
+        assert len(table) == 1
+        e = table[0]
+        assert (e["Nx"], e["Ny"], e["Np"], e["Nfft"]) == (grid.Nx, grid.Ny, geom.Np, cfg.nfft)
+        assert (e["tile"], e["pulse_split"]) == (res["tile"], res["pulse_split"])
+        # a pair that differs from the untuned default (tile 11 x 2 splits, or 1 x 2)
+        e["tile"], e["pulse_split"] = (1, 2) if res["tile"] == 11 else (11, 2)
+        bad = dict(e, tile=99)
+        with pytest.raises(_lib.SarfError) as err:
+            b_ctx.load_tune_table([e, bad])
+        assert err.value.code == _lib.SARF_ERR_INVALID
+        assert b_ctx.tune_table() == []
+        b_ctx.load_tune_table([e])
+        assert b_ctx.tune_table() == [e]
+        dflt = b_ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr).cpu()
+        expl = b_ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr, tile=e["tile"],
+                                 pulse_split=e["pulse_split"]).cpu()
+        assert torch.equal(dflt, expl)
+        assert not torch.equal(dflt, untuned)
+        ref = oracle.bp_f64(oracle.range_compress_f64(ph, cfg.nfft), geom.ant, geom.r0, geom.f0,
+                            grid, geom.maxWr)
+        assert oracle.rel_l2(dflt.numpy(), ref) <= 1e-4
+    finally:
+        a_ctx.close()
+        b_ctx.close()
+
+
 # ------------------------------------------------------------ matched filter
 @pytest.mark.parametrize("Np,K,Nx,Ny,split", [(24, 96, 40, 36, 0), (8, 1501, 20, 70, 0),
                                               (33, 64, 17, 130, 5), (3, 1, 5, 3, 0)])

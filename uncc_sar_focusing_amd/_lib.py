@@ -80,6 +80,8 @@ SIGNATURES = {
     "sarf_synth_phase_history_nufft": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _vp,
                                             _vp]),
     "sarf_autotune": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp]),
+    "sarf_tune_table_get": (_i, [_vp, C.c_void_p, _i, C.POINTER(_i)]),
+    "sarf_tune_table_set": (_i, [_vp, C.c_void_p, _i]),
     "sarf_matched_filter": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sarf_image_db8": (_i, [_vp, _vp, _i, _i, C.c_float, _vp, _vp]),
     "sarf_autotune_scenes": (_i, [_vp, _i, C.c_void_p, _i, _vp, _vp, _vp]),
@@ -98,6 +100,13 @@ class MfDesc(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("tile", C.c_int), ("pulse_split", C.c_int), ("ms", C.c_float),
                 ("default_ms", C.c_float), ("candidates", C.c_int)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("Np", C.c_int), ("Nfft", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("maxWr", C.c_double),
+                ("los_x", C.c_double), ("los_y", C.c_double),
+                ("tile", C.c_int), ("pulse_split", C.c_int)]
 
 
 class SceneDesc(C.Structure):

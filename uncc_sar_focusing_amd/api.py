@@ -217,6 +217,27 @@ class Context:
         res = {k: getattr(r, k) for k, _ in _lib.TuneResult._fields_}
         return res, out
 
+    def tune_table(self) -> list:
+        """The launch shapes `autotune` remembered on this context, as a list
+        of dicts (geometry: Nx, Ny, Np, Nfft, dx, dy, maxWr, los_x, los_y;
+        choice: tile, pulse_split) that `json.dump` can write."""
+        n = C.c_int(0)
+        check(self.lib.sarf_tune_table_get(self._h, None, 0, C.byref(n)))
+        arr = (_lib.TuneEntry * max(n.value, 1))()
+        check(self.lib.sarf_tune_table_get(self._h, arr, n.value, C.byref(n)))
+        return [{k: getattr(arr[i], k) for k, _ in _lib.TuneEntry._fields_}
+                for i in range(n.value)]
+
+    def load_tune_table(self, entries) -> None:
+        """Adds the shapes of `entries` (as `tune_table` returns them, e.g.
+        read back with `json.load`) to this context; later `backproject`
+        calls with tile=0, pulse_split=0 on a matching geometry use them."""
+        entries = list(entries)
+        arr = (_lib.TuneEntry * max(len(entries), 1))()
+        for i, e in enumerate(entries):
+            arr[i] = _lib.TuneEntry(*(e[k] for k, _ in _lib.TuneEntry._fields_))
+        check(self.lib.sarf_tune_table_set(self._h, arr, len(entries)))
+
     # ------------------------------------------------------------- entropy
     def image_entropy(self, img: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """Device tensor [E, S, T] (float64): E = -sum p ln p, p = |I|^2/S."""

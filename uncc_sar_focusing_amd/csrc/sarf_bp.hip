@@ -248,16 +248,33 @@ int bp_fill_args(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, BpArgs *out) {
 
 namespace {
 
-std::array<uint64_t, 8> tune_key(const sarf_bp_desc *d) {
+// The geometry a remembered launch shape applies to (spacings as the
+// backprojection computes them, W / (N - 1)).
+sarf_tune_entry tune_geometry(const sarf_bp_desc *d) {
+    sarf_tune_entry e{};
+    e.Nx = d->Nx;
+    e.Ny = d->Ny;
+    e.Np = d->Np;
+    e.Nfft = d->Nfft;
+    e.dx = d->Wx / (d->Nx - 1);
+    e.dy = d->Wy / (d->Ny - 1);
+    e.maxWr = d->maxWr;
+    e.los_x = d->los_x;
+    e.los_y = d->los_y;
+    return e;
+}
+
+std::array<uint64_t, 8> tune_key(const sarf_tune_entry &e) {
     auto bits = [](double v) {
         uint64_t u;
         std::memcpy(&u, &v, sizeof u);
         return u;
     };
-    return {(uint64_t)d->Nx, (uint64_t)d->Ny, (uint64_t)d->Np, (uint64_t)d->Nfft,
-            bits(d->Wx / (d->Nx - 1)), bits(d->Wy / (d->Ny - 1)), bits(d->maxWr),
-            bits(d->los_x) * 31 + bits(d->los_y)};
+    return {(uint64_t)e.Nx, (uint64_t)e.Ny, (uint64_t)e.Np, (uint64_t)e.Nfft,
+            bits(e.dx), bits(e.dy), bits(e.maxWr), bits(e.los_x) * 31 + bits(e.los_y)};
 }
+
+std::array<uint64_t, 8> tune_key(const sarf_bp_desc *d) { return tune_key(tune_geometry(d)); }
 
 // Default pulse splits: split the pulses into groups that form partial
 // images (summed in order afterwards) until there are 64 workgroups per CU,
@@ -351,8 +368,8 @@ int launch_backproject(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof, c
         std::lock_guard<std::mutex> lk(ctx->mu);
         auto it = ctx->tuned.find(tune_key(d));
         if (it != ctx->tuned.end()) {
-            tile = it->second.first;
-            splits = it->second.second;
+            tile = it->second.tile;
+            splits = it->second.pulse_split;
         }
     }
     if (tile == 0) {
@@ -453,12 +470,48 @@ extern "C" int sarf_autotune(sarf_ctx *ctx, const sarf_bp_desc *d, const void *p
     SARF_HIP(hipStreamSynchronize(s));
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
-        ctx->tuned[tune_key(d)] = {best_tile, best_split};
+        sarf_tune_entry e = tune_geometry(d);
+        e.tile = best_tile;
+        e.pulse_split = best_split;
+        ctx->tuned[tune_key(e)] = e;
     }
     out->tile = best_tile;
     out->pulse_split = best_split;
     out->ms = best_ms;
     out->default_ms = def_ms;
     out->candidates = n;
+    return SARF_OK;
+}
+
+extern "C" int sarf_tune_table_get(sarf_ctx *ctx, sarf_tune_entry *out, int cap, int *count) {
+    clear_error();
+    if (!ctx || !count || cap < 0 || (cap > 0 && !out))
+        return fail(SARF_ERR_INVALID, "sarf_tune_table_get: bad arguments");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    int i = 0;
+    for (const auto &kv : ctx->tuned) {
+        if (i < cap) out[i] = kv.second;
+        i++;
+    }
+    *count = i;
+    return SARF_OK;
+}
+
+extern "C" int sarf_tune_table_set(sarf_ctx *ctx, const sarf_tune_entry *entries, int n) {
+    clear_error();
+    if (!ctx || n < 0 || (n > 0 && !entries))
+        return fail(SARF_ERR_INVALID, "sarf_tune_table_set: bad arguments");
+    for (int i = 0; i < n; i++) {
+        const sarf_tune_entry &e = entries[i];
+        int tx = 0, ty = 0;
+        if (e.Nx < 2 || e.Ny < 2 || e.Np < 1 || e.Nfft < 2 || !(e.dx > 0) || !(e.dy > 0) ||
+            !(e.maxWr > 0))
+            return fail(SARF_ERR_INVALID, "sarf_tune_table_set: entry %d has a bad geometry", i);
+        if (!bp_tile_shape(e.tile, &tx, &ty) || e.pulse_split < 1 || e.pulse_split > e.Np)
+            return fail(SARF_ERR_INVALID, "sarf_tune_table_set: entry %d: tile %d / pulse_split %d",
+                        i, e.tile, e.pulse_split);
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (int i = 0; i < n; i++) ctx->tuned[tune_key(entries[i])] = entries[i];
     return SARF_OK;
 }

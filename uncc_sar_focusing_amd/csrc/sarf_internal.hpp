@@ -61,7 +61,7 @@ struct sarf_ctx {
     std::map<std::string, sarf::Scratch> scratch;
     // backprojection launch shapes chosen by sarf_autotune, keyed by the
     // image geometry: (tile id, pulse splits)
-    std::map<std::array<uint64_t, 8>, std::pair<int, int>> tuned;
+    std::map<std::array<uint64_t, 8>, sarf_tune_entry> tuned;
     // last scene list uploaded by sarf_bp_scenes (caller's descriptors and
     // tile), so an identical call re-launches without a copy or a wait
     std::vector<unsigned char> scenes_key;
