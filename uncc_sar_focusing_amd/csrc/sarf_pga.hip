@@ -464,13 +464,19 @@ __device__ __forceinline__ void bluestein_regs(float2 (&v)[16], float2 *Lr, int 
     fft::ifft_regs<LOG2M>(v, Lr, j, twM);
 }
 
-// Step 3 for the P rows of block blk (M/16 lanes per row, M = 2^LOG2M), as
-// row_dft_lds_body.
+// LDS of the register-resident row kernels: P padded rows.  The kernels
+// declare it once for both row steps, so a block holds one copy.
 template <int LOG2M>
-__device__ __forceinline__ void row_dft_body(const PgaImage &p, int blk) {
+constexpr int row_lds_elems() {
+    return fft::rows_per_group(LOG2M) * fft::padded(LOG2M);
+}
+
+// Step 3 for the P rows of block blk (M/16 lanes per row, M = 2^LOG2M), as
+// row_dft_lds_body; lds: row_lds_elems<LOG2M>() elements.
+template <int LOG2M>
+__device__ __forceinline__ void row_dft_body(const PgaImage &p, int blk, float2 *lds) {
     using S = fft::Shape<LOG2M>;
     constexpr int M = S::N, T = S::T, P = fft::rows_per_group(LOG2M), PAD = fft::padded(LOG2M);
-    __shared__ float2 lds[P * PAD];
     const int N = p.Nx;
     const int lp = threadIdx.x / T, j = threadIdx.x % T;
     const int y = blk * P + lp;
@@ -513,10 +519,9 @@ __device__ __forceinline__ void row_dft_body(const PgaImage &p, int blk) {
 
 // Step 6 for the P rows of block blk, in place, as row_apply_lds_body.
 template <int LOG2M>
-__device__ __forceinline__ void row_apply_body(const PgaImage &p, int blk) {
+__device__ __forceinline__ void row_apply_body(const PgaImage &p, int blk, float2 *lds) {
     using S = fft::Shape<LOG2M>;
     constexpr int M = S::N, T = S::T, P = fft::rows_per_group(LOG2M), PAD = fft::padded(LOG2M);
-    __shared__ float2 lds[P * PAD];
     const int N = p.Nx;
     const int lp = threadIdx.x / T, j = threadIdx.x % T;
     const int y = blk * P + lp;
@@ -601,8 +606,9 @@ __global__ __launch_bounds__(kThreads) void k1_rows_lds(PgaImage p, int apply) {
 }
 template <int LOG2M>
 __global__ __launch_bounds__(fft::group_threads(LOG2M)) void k1_rows(PgaImage p, int apply) {
-    if (apply) row_apply_body<LOG2M>(p, blockIdx.x);
-    else row_dft_body<LOG2M>(p, blockIdx.x);
+    __shared__ float2 lds[row_lds_elems<LOG2M>()];
+    if (apply) row_apply_body<LOG2M>(p, blockIdx.x, lds);
+    else row_dft_body<LOG2M>(p, blockIdx.x, lds);
 }
 
 // ------------------------------------------------ kernels: many images
@@ -679,8 +685,9 @@ __global__ __launch_bounds__(fft::group_threads(LOG2M)) void kb_rows(PgaBatchDev
     const int *ids = b.grp + g0, *off = ids + m;
     const int k = owner(off, m, blockIdx.x);
     const PgaImage &p = b.img[ids[k]];
-    if (apply) row_apply_body<LOG2M>(p, blockIdx.x - off[k]);
-    else row_dft_body<LOG2M>(p, blockIdx.x - off[k]);
+    __shared__ float2 lds[row_lds_elems<LOG2M>()];
+    if (apply) row_apply_body<LOG2M>(p, blockIdx.x - off[k], lds);
+    else row_dft_body<LOG2M>(p, blockIdx.x - off[k], lds);
 }
 
 // ---------------------------------------------------------------- host
