@@ -49,6 +49,7 @@ extern "C" {
 
 /* backprojection flags */
 #define SARF_BP_ACCUMULATE 1 /* add into img instead of overwriting it */
+#define SARF_BATCH_DIR_PER_BASIS 2 /* sarf_bp_batch_entropy: dir holds one field per basis */
 
 /* range-compression flags */
 #define SARF_RC_TAPER 1 /* multiply sample k by taper[k] (rangeCompress.m:6-12) */
@@ -247,6 +248,11 @@ int sarf_image_entropy(sarf_ctx *ctx, const void *img, /* device complex, npix *
 /* (e.g. a line-of-sight error a t^2 + b t^3 + c t^4), backprojects all     */
 /* pulses (mixed precision, bp desc as for sarf_backproject) and writes the  */
 /* entropy of candidate c's image to entropy[c]; images are not stored.     */
+/* With desc->flags = SARF_BATCH_DIR_PER_BASIS, dir is nbasis x Np x 3 and  */
+/*   a_n(c) = a_n + sum_k coef[c*nbasis + k] * basis[k*Np + n] * dir[k][n],  */
+/* which with a_n = 0 and bases 1, i, i^2 along x, y, z expresses the       */
+/* reference's absolute candidate trajectories (D = 7 and D = 10,           */
+/* src/autofocus/gridSearchErrorFunctions_dft.cuh:279-291).                 */
 /* Replaces kernelWrapper (src/autofocus/gridSearchErrorFunctions_dft.cuh:  */
 /* 258-336) driven by cuGridSearch (src/autofocus/autofocus_gpu.cu:         */
 /* 1057-1070).                                                              */
@@ -257,7 +263,7 @@ int sarf_bp_batch_entropy(sarf_ctx *ctx, const sarf_bp_desc *desc,
                           int ncand, int nbasis,
                           const double *coef,   /* device, ncand x nbasis */
                           const double *basis,  /* device, nbasis x Np */
-                          const double *dir,    /* device, Np x 3 unit vectors */
+                          const double *dir,    /* device, Np x 3 (nbasis x Np x 3 per basis) */
                           double *entropy,      /* device, ncand */
                           void *stream);
 

@@ -195,6 +195,25 @@ def db8(img, dyn_range_db: float = 70.0) -> np.ndarray:
     return out
 
 
+def reference_trajectory_f64(params, D: int, Np: int) -> np.ndarray:
+    """(Np, 3) candidate phase centres of the reference's grid-search model
+    (src/autofocus/gridSearchErrorFunctions_dft.cuh:279-291), pulse index
+    i = 0..Np-1, evaluated in FP64:
+      D = 10: x_i = p2 i^2 + p1 i p9 + p0, y_i = p5 i^2 + p4 i p9 + p3,
+              z_i = p8 i^2 + p7 i p9 + p6;
+      D = 7:  x_i = p1 i p6 + p0, y_i = p3 i p6 + p2, z_i = p5 i p6 + p4.
+    Pinned by construction against those lines (tests/test_workloads.py
+    checks the engine's linear form against it)."""
+    p = np.asarray(params, dtype=np.float64)
+    i = np.arange(Np, dtype=np.float64)
+    if D == 10:
+        return np.stack([p[3 * a + 2] * i * i + p[3 * a + 1] * i * p[9] + p[3 * a]
+                         for a in range(3)], axis=1)
+    if D == 7:
+        return np.stack([p[2 * a + 1] * i * p[6] + p[2 * a] for a in range(3)], axis=1)
+    raise ValueError("D must be 7 or 10")
+
+
 def entropy_f64(img) -> float:
     img = _c128(img)
     return lib().orc_entropy_f64(_p(img), img.size)

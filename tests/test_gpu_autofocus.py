@@ -41,6 +41,41 @@ def _los_scene(Np=48, K=64, n=40, spacing=0.12, true_coef=(0.02, -0.015, 0.01)):
     return geom, ph, grid
 
 
+def test_batch_entropy_reference_trajectory_model(ctx):
+    """The reference's own candidate model (absolute positions quadratic in
+    the pulse index, D = 10, gridSearchErrorFunctions_dft.cuh:279-291) through
+    the per-basis form: each candidate's batched entropy equals the
+    single-image path's at those positions (1e-5 relative) and the FP64
+    oracle's (1e-4 absolute), with the same argmin."""
+    geom, ph, grid = _los_scene()
+    nfft = 512
+    i = np.arange(geom.Np, dtype=np.float64)
+    base = np.zeros(10)
+    for a in range(3):  # least-squares quadratic in i per axis: (c0, c1, c2)
+        c2, c1, c0 = np.polyfit(i, geom.ant[:, a], 2)
+        base[3 * a:3 * a + 3] = (c0, c1, c2)
+    base[9] = 1.0
+    params = np.repeat(base[None, :], 7, axis=0)
+    params[:, 5] += np.linspace(-2e-5, 2e-5, 7)   # y curvature
+    params[1::2, 9] *= 1.0 + 1e-5                 # velocity scale
+    basis, direction, coefs = af.reference_trajectory_candidates(params, 10, geom.Np)
+    prof_h = oracle.range_compress_f64(ph, nfft)
+    prof = to_dev(prof_h.astype(np.complex64))
+    r0, f0 = to_dev(geom.r0), to_dev(geom.f0)
+    ent = ctx.batch_entropy(prof, to_dev(np.zeros_like(geom.ant)), r0, f0, grid, geom.maxWr,
+                            to_dev(coefs), to_dev(basis), to_dev(direction)).cpu().numpy()
+    ref = []
+    for j in range(len(params)):
+        pos = oracle.reference_trajectory_f64(params[j], 10, geom.Np)
+        img = ctx.backproject(prof, to_dev(pos), r0, f0, grid, geom.maxWr)
+        e = float(ctx.image_entropy(img).cpu()[0])
+        assert abs(ent[j] - e) <= 1e-5 * abs(e), (j, ent[j], e)
+        ref.append(oracle.entropy_f64(oracle.bp_f64(prof_h, pos, geom.r0, geom.f0, grid,
+                                                    geom.maxWr)))
+    assert np.max(np.abs(ent - np.array(ref))) < 1e-4
+    assert int(np.argmin(ent)) == int(np.argmin(ref))
+
+
 @pytest.mark.parametrize("tile", [0, 4, 11, 12, 13, 14])
 def test_batch_entropy_matches_oracle_per_candidate(ctx, tile):
     """Every batch tile shape, both lane orientations (0 = default choice)."""

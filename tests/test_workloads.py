@@ -45,3 +45,40 @@ def test_configs_match_the_workload_sizes():
     assert abs(np.linalg.norm(g.ant[0]) - 12500.0) < 1e-6
     span = np.degrees(np.arctan2(g.ant[-1, 1], g.ant[-1, 0]) - np.arctan2(g.ant[0, 1], g.ant[0, 0]))
     assert abs(abs(span) - 4.0) < 1e-9
+
+
+def test_reference_trajectory_oracle_closed_forms():
+    """The oracle's restatement of the reference's candidate model
+    (gridSearchErrorFunctions_dft.cuh:279-291) against what the formulas fix:
+    pulse 0 sits at the constant terms; D = 7 moves by the constant step
+    (p1, p3, p5) * p6 per pulse; D = 10 has constant second differences
+    2 (p2, p5, p8) and first step (p1, p4, p7) * p9 + (p2, p5, p8)."""
+    import oracle
+
+    rng = np.random.default_rng(5)
+    p7 = rng.standard_normal(7) * 100.0
+    a = oracle.reference_trajectory_f64(p7, 7, 50)
+    assert np.allclose(a[0], p7[[0, 2, 4]])
+    assert np.allclose(np.diff(a, axis=0), (p7[[1, 3, 5]] * p7[6])[None, :])
+    p10 = rng.standard_normal(10) * 10.0
+    b = oracle.reference_trajectory_f64(p10, 10, 50)
+    assert np.allclose(b[0], p10[[0, 3, 6]])
+    assert np.allclose(np.diff(b, 2, axis=0), 2.0 * p10[[2, 5, 8]][None, :])
+    assert np.allclose(b[1] - b[0], p10[[1, 4, 7]] * p10[9] + p10[[2, 5, 8]])
+
+
+def test_reference_trajectory_candidates_reproduce_the_model():
+    """The per-basis linear form that batch_entropy takes reproduces the
+    reference's D = 7 and D = 10 candidate positions (FP64)."""
+    import oracle
+    from uncc_sar_focusing_amd.autofocus import reference_trajectory_candidates
+
+    rng = np.random.default_rng(9)
+    for D in (7, 10):
+        params = rng.standard_normal((5, D)) * 50.0
+        basis, direction, coefs = reference_trajectory_candidates(params, D, 300)
+        assert basis.shape[0] == direction.shape[0] == coefs.shape[1] == (6 if D == 7 else 9)
+        for j in range(5):
+            pos = np.einsum("k,kn,knx->nx", coefs[j], basis, direction)
+            ref = oracle.reference_trajectory_f64(params[j], D, 300)
+            assert np.max(np.abs(pos - ref)) <= 1e-12 * np.max(np.abs(ref))

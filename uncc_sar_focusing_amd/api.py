@@ -258,7 +258,9 @@ class Context:
                       los: tuple[float, float] | None = None,
                       out: torch.Tensor | None = None) -> torch.Tensor:
         """Entropy of the image of every candidate trajectory
-        a_n + (coef[c] . basis[:, n]) * direction[n], one persistent launch."""
+        a_n + (coef[c] . basis[:, n]) * direction[n], one persistent launch.
+        `direction` may instead be (nbasis, Np, 3), one field per basis:
+        a_n + sum_k coef[c, k] basis[k, n] direction[k, n]."""
         Np, nfft = prof.shape
         ncand, nb = coef.shape
         _need(prof, torch.complex64, (Np, nfft), "prof")
@@ -267,12 +269,14 @@ class Context:
         _need(f0, torch.float64, (Np,), "f0")
         _need(coef, torch.float64, (ncand, nb), "coef")
         _need(basis, torch.float64, (nb, Np), "basis")
-        _need(direction, torch.float64, (Np, 3), "direction")
+        per_basis = direction.dim() == 3
+        _need(direction, torch.float64, (nb, Np, 3) if per_basis else (Np, 3), "direction")
         if out is None:
             out = torch.empty(ncand, dtype=torch.float64, device=prof.device)
         _need(out, torch.float64, (ncand,), "out")
         d = BpDesc(grid.Nx, grid.Ny, nfft, Np, grid.x0, grid.y0, grid.Wx, grid.Wy, maxWr,
-                   _lib.SARF_BP_MIXED, 0, tile, 0, *(los if los is not None else (0.0, 0.0)))
+                   _lib.SARF_BP_MIXED, _lib.SARF_BATCH_DIR_PER_BASIS if per_basis else 0, tile, 0,
+                   *(los if los is not None else (0.0, 0.0)))
         check(self.lib.sarf_bp_batch_entropy(self._h, C.byref(d), _ptr(prof), _ptr(ant),
                                              _ptr(r0), _ptr(f0), ncand, nb, _ptr(coef),
                                              _ptr(basis), _ptr(direction), _ptr(out),

@@ -38,6 +38,34 @@ def apply_los_offsets(ant: np.ndarray, coef, basis: np.ndarray, direction: np.nd
     return ant + off[:, None] * direction
 
 
+def reference_trajectory_candidates(params, D: int, Np: int):
+    """The reference's grid-search candidates -- absolute phase centres that
+    are polynomials in the pulse index i along x, y and z
+    (src/autofocus/gridSearchErrorFunctions_dft.cuh:279-291; D = 7: linear
+    with a shared velocity scale p6; D = 10: quadratic with scale p9) -- as
+    the linear per-basis form of batch_entropy: with antenna positions 0,
+    a_i = sum_k coef_k basis_k[i] dir_k.  params: (ncand, D).  Returns
+    (basis (nb, Np), direction (nb, Np, 3), coefs (ncand, nb)); nb = 6 for
+    D = 7 (1, i per axis), 9 for D = 10 (1, i, i^2 per axis)."""
+    p = np.atleast_2d(np.asarray(params, dtype=np.float64))
+    if p.shape[1] != D or D not in (7, 10):
+        raise ValueError("params must be (ncand, 7) or (ncand, 10)")
+    i = np.arange(Np, dtype=np.float64)
+    powers = (0, 1) if D == 7 else (0, 1, 2)
+    nb = 3 * len(powers)
+    basis = np.empty((nb, Np))
+    direction = np.zeros((nb, Np, 3))
+    coefs = np.empty((p.shape[0], nb))
+    scale = p[:, 6] if D == 7 else p[:, 9]
+    for a in range(3):
+        for q, pw in enumerate(powers):
+            k = a * len(powers) + q
+            basis[k] = i ** pw
+            direction[k, :, a] = 1.0
+            coefs[:, k] = p[:, a * len(powers) + q] * (scale if pw == 1 else 1.0)
+    return basis, direction, coefs
+
+
 def coef_grid(lo: float, hi: float, n: int, dims: int = 3) -> np.ndarray:
     """n^dims candidates on a regular grid over [lo, hi]^dims (first
     coefficient slowest)."""

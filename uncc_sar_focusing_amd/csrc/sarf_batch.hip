@@ -10,7 +10,10 @@
 //
 // Here: candidate c moves every phase centre along a unit direction,
 //   a_n(c) = a_n + (sum_k coef[c,k] basis[k,n]) dir_n,
-// e.g. a line-of-sight polynomial error a t^2 + b t^3 + c t^4.  A persistent
+// e.g. a line-of-sight polynomial error a t^2 + b t^3 + c t^4, or, with one
+// direction field per basis, a_n(c) = a_n + sum_k coef[c,k] basis[k,n] dir[k,n],
+// which with a_n = 0 and bases 1, i, i^2 along x, y, z is the reference's own
+// absolute trajectory model (:279-291).  A persistent
 // grid walks (tile, candidate) work items, tile-major, so the workgroups
 // resident at one time share a tile and read nearly the same profile windows
 // (L2 reuse).  Each item backprojects all pulses into registers with the
@@ -37,7 +40,7 @@ void bp_batch_entropy(
     BpArgs a, const float2 *__restrict__ prof, const double *__restrict__ ant,
     const double *__restrict__ r0, const double *__restrict__ f0, const double *__restrict__ dir,
     const double *__restrict__ basis, const double *__restrict__ coef, int nbasis, int ncand,
-    double2 *__restrict__ partial) {
+    int per_basis, double2 *__restrict__ partial) {
     __shared__ BpLds L;
     __shared__ __align__(16) float fx[BpTile<TY, PX, XL>::kFx];
     __shared__ double2 red[kBpThreads / 64];
@@ -48,7 +51,7 @@ void bp_batch_entropy(
         const int c = (int)(item - (long)t * ncand);
         BpTile<TY, PX, XL> T;
         T.init(a, t);
-        LosPolyPos pos{ant, dir, basis, coef + (size_t)c * nbasis, nbasis, a.Np};
+        LosPolyPos pos{ant, dir, basis, coef + (size_t)c * nbasis, nbasis, a.Np, per_basis};
         T.run(a, L, fx, prof, pos, r0, f0, 0, a.Np);
         double s = 0.0, tt = 0.0;
 #pragma unroll
@@ -121,9 +124,10 @@ __global__ __launch_bounds__(256) void image_entropy_final(const double2 *__rest
 template <int TY, int PX, bool XL = false>
 void launch_batch_t(const BpArgs &a, int grid, const float2 *prof, const double *ant,
                     const double *r0, const double *f0, const double *dir, const double *basis,
-                    const double *coef, int nbasis, int ncand, double2 *partial, hipStream_t s) {
+                    const double *coef, int nbasis, int ncand, int per_basis, double2 *partial,
+                    hipStream_t s) {
     hipLaunchKernelGGL((bp_batch_entropy<TY, PX, XL>), dim3(grid), dim3(kBpThreads), 0, s, a,
-                       prof, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial);
+                       prof, ant, r0, f0, dir, basis, coef, nbasis, ncand, per_basis, partial);
 }
 
 }  // namespace
@@ -144,6 +148,13 @@ extern "C" int sarf_bp_batch_entropy(sarf_ctx *ctx, const sarf_bp_desc *d, const
         return fail(SARF_ERR_INVALID, "sarf_bp_batch_entropy: ncand=%d nbasis=%d", ncand, nbasis);
     if (d->precision != SARF_BP_MIXED)
         return fail(SARF_ERR_UNSUPPORTED, "sarf_bp_batch_entropy: mixed precision only");
+    if (d->flags & ~SARF_BATCH_DIR_PER_BASIS)
+        return fail(SARF_ERR_INVALID, "sarf_bp_batch_entropy: flags %d (only "
+                    "SARF_BATCH_DIR_PER_BASIS)", d->flags);
+    const int per_basis = (d->flags & SARF_BATCH_DIR_PER_BASIS) ? 1 : 0;
+    sarf_bp_desc dd = *d;
+    dd.flags = 0;
+    d = &dd;
     BpArgs a{};
     const int tile = d->tile > 0 ? d->tile : 0;
     SARF_TRY(bp_fill_args(ctx, d, tile, &a));
@@ -160,15 +171,15 @@ extern "C" int sarf_bp_batch_entropy(sarf_ctx *ctx, const sarf_bp_desc *d, const
     hipStream_t s = (hipStream_t)stream;
     const float2 *p = (const float2 *)prof;
     switch (tid) {
-        case 1: launch_batch_t<32, 4>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
-        case 2: launch_batch_t<64, 4>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
-        case 3: launch_batch_t<64, 8>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
-        case 4: launch_batch_t<32, 8>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
-        case 5: launch_batch_t<16, 4>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
-        case 11: launch_batch_t<32, 4, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
-        case 12: launch_batch_t<64, 4, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
-        case 13: launch_batch_t<64, 8, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
-        case 14: launch_batch_t<32, 8, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, partial, s); break;
+        case 1: launch_batch_t<32, 4>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, per_basis, partial, s); break;
+        case 2: launch_batch_t<64, 4>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, per_basis, partial, s); break;
+        case 3: launch_batch_t<64, 8>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, per_basis, partial, s); break;
+        case 4: launch_batch_t<32, 8>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, per_basis, partial, s); break;
+        case 5: launch_batch_t<16, 4>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, per_basis, partial, s); break;
+        case 11: launch_batch_t<32, 4, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, per_basis, partial, s); break;
+        case 12: launch_batch_t<64, 4, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, per_basis, partial, s); break;
+        case 13: launch_batch_t<64, 8, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, per_basis, partial, s); break;
+        case 14: launch_batch_t<32, 8, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, per_basis, partial, s); break;
         default:
             return fail(SARF_ERR_UNSUPPORTED, "sarf_bp_batch_entropy: tile %d not compiled for batches", tid);
     }

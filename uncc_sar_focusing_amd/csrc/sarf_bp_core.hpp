@@ -98,11 +98,25 @@ struct PlainPos {
 };
 
 // Candidate trajectory: a_n + (sum_k coef_k basis_k[n]) dir_n -- e.g. a
-// polynomial line-of-sight error a t^2 + b t^3 + c t^4 along the unit LOS.
+// polynomial line-of-sight error a t^2 + b t^3 + c t^4 along the unit LOS --
+// or, with one direction field per basis (per_basis),
+// a_n + sum_k coef_k basis_k[n] dir_k,n -- e.g. absolute positions that are
+// polynomials in the pulse index along x, y and z.
 struct LosPolyPos {
     const double *ant, *dir, *basis, *coef;
-    int nbasis, Np;
+    int nbasis, Np, per_basis;
     __device__ __forceinline__ double3 operator()(int n) const {
+        if (per_basis) {
+            double x = ant[3 * n], y = ant[3 * n + 1], z = ant[3 * n + 2];
+            for (int k = 0; k < nbasis; k++) {
+                const double w = coef[k] * basis[(size_t)k * Np + n];
+                const double *d = dir + 3 * ((size_t)k * Np + n);
+                x = fma(w, d[0], x);
+                y = fma(w, d[1], y);
+                z = fma(w, d[2], z);
+            }
+            return make_double3(x, y, z);
+        }
         double off = 0.0;
         for (int k = 0; k < nbasis; k++) off = fma(coef[k], basis[(size_t)k * Np + n], off);
         return make_double3(fma(off, dir[3 * n], ant[3 * n]), fma(off, dir[3 * n + 1], ant[3 * n + 1]),
