@@ -1,7 +1,8 @@
 """CPU oracle of the SAR focusing path -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
-this package, and only as the checker (or as the timed CPU baseline).  The
+this package, and only as the reference results that GPU outputs are compared
+with (or as the timed CPU baseline).  The
 product path (uncc_sar_focusing_amd + libsarf.so) never imports it.
 
 Numerics live in sarf_oracle.c (C, FP64 and an FP32-faithful restatement of

@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle and the CPU
  * baseline.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
- * leg may load it, and only as the checker (or as the timed CPU baseline),
+ * leg may load it, and only as the reference results that GPU outputs are
+ * compared with (or as the timed CPU baseline),
  * never as the product path.  The product is the HIP library under
  * uncc_sar_focusing_amd/csrc, which has no CPU fallback.
  *
