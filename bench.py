@@ -293,8 +293,8 @@ def main() -> int:
                     help="only measure FP32 FMA, v_sin/v_cos and copy rates and print them")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); "
+              "sharding over several GPUs is designed but not implemented", file=sys.stderr)
         return 2
 
     if args.calibrate_copy or args.peaks:
