@@ -260,10 +260,19 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const d
         // back in chunks, each copied out of pinned staging while the next is
         // in flight.  Backprojection is additive over pulses, so each group
         // accumulates into the image.
-        const int G = std::max(1, std::min(4, d->Np / 1024));
+        // The first group's upload is not overlapped with anything, so it is
+        // kept short (an eighth of the pulses); the rest split evenly.
+        const int Gr = std::max(1, std::min(4, d->Np / 1024));
+        const int G = Gr > 1 ? Gr + 1 : 1;
+        const int first = G > 1 ? d->Np / 8 : d->Np;
+        auto group_start = [&](int g) {
+            if (g == 0) return 0;
+            if (g >= G) return d->Np;
+            return first + (int)((long)(d->Np - first) * (g - 1) / (G - 1));
+        };
         size_t gmax = 0;
         for (int g = 0; g < G; g++) {
-            const int n0 = (int)((long)d->Np * g / G), n1 = (int)((long)d->Np * (g + 1) / G);
+            const int n0 = group_start(g), n1 = group_start(g + 1);
             gmax = std::max(gmax, sizeof(float2) * (size_t)(n1 - n0) * d->K);
         }
         const size_t slot = (gmax + 255) & ~(size_t)255, img_bytes = img_elems * csz;
@@ -287,7 +296,7 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const d
             if (hipEventCreateWithFlags(&dl[c], hipEventDisableTiming) != hipSuccess)
                 rc = fail(SARF_ERR_HIP, "sarf_focus: event creation failed");
         for (int g = 0; g < G && rc == SARF_OK; g++) {
-            const int n0 = (int)((long)d->Np * g / G), n1 = (int)((long)d->Np * (g + 1) / G);
+            const int n0 = group_start(g), n1 = group_start(g + 1);
             const size_t off = (size_t)n0 * d->K, bytes = sizeof(float2) * (size_t)(n1 - n0) * d->K;
             const int k = g & 1;
             // staging slot k was last sent for group g - 2
