@@ -195,6 +195,25 @@ def test_bp_every_tile_shape(ctx, tile):
     assert local_peaks(img, grid, tg.pos) == local_peaks(ref, grid, tg.pos)
 
 
+@pytest.mark.parametrize("K,nfft", [(96, 96), (96, 300), (424, 424)])
+def test_bp_non_power_of_two_nfft_matches_oracle(ctx, K, nfft):
+    """Nfft = K, the reference's default (charlotte_sar_api.hpp:173), and other
+    lengths that are not powers of two: direct-DFT range compression, then
+    mixed BP (every bin width and window follow Nfft), and the host-buffer
+    path, against the FP64 oracle.  Tolerance: rel-L2 <= 1e-4, same peaks."""
+    geom, tg, ph = small_inputs(Np=64, K=K)
+    grid = ImageGrid(48, 40, 0.2, -0.1, 6.0, 5.0)
+    ant, r0, f0 = geom_dev(geom)
+    prof = ctx.range_compress(to_dev(ph), nfft)
+    img = ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr).cpu().numpy()
+    ref = oracle.bp_f64(oracle.range_compress_f64(ph, nfft), geom.ant, geom.r0, geom.f0, grid,
+                        geom.maxWr)
+    assert oracle.rel_l2(img, ref) <= 1e-4
+    assert local_peaks(img, grid, tg.pos) == local_peaks(ref, grid, tg.pos)
+    host = ctx.focus_host(ph, geom.ant, geom.r0, geom.f0, grid, nfft, geom.maxWr)
+    assert oracle.rel_l2(host, ref) <= 1e-4
+
+
 @pytest.mark.parametrize("Np", [1, 31, 33, 100])
 def test_bp_pulse_counts(ctx, Np):
     geom, tg, ph = small_inputs(Np=Np, K=64)
