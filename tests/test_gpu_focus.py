@@ -295,6 +295,25 @@ def test_bp_is_deterministic(ctx):
     assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("tile", [0, 1, 4, 7, 11, 13])
+def test_bp_smallest_images(ctx, tile):
+    """Images far smaller than a tile (2x2, 2x3, 3x2, 7x2) centred on a
+    target's main lobe: the tile's pixels outside the image are neither
+    computed into nor written; rel-L2 <= 1e-4 against the oracle."""
+    geom, tg, ph = small_inputs(Np=20, K=64)
+    ant, r0, f0 = geom_dev(geom)
+    nfft = 512
+    prof = ctx.range_compress(to_dev(ph), nfft)
+    prof_h = oracle.range_compress_f64(ph, nfft)
+    cx, cy = tg.pos[0, 0], tg.pos[0, 1]
+    for nx, ny in [(2, 2), (2, 3), (3, 2), (7, 2)]:
+        grid = ImageGrid(nx, ny, cx, cy, 0.03 * (nx - 1), 0.03 * (ny - 1))
+        img = ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr, tile=tile).cpu().numpy()
+        ref = oracle.bp_f64(prof_h, geom.ant, geom.r0, geom.f0, grid, geom.maxWr)
+        assert img.shape == (nx, ny)
+        assert oracle.rel_l2(img, ref) <= 1e-4, (tile, nx, ny)
+
+
 def test_bp_rejects_bad_descriptor(ctx):
     geom, tg, ph = small_inputs()
     ant, r0, f0 = geom_dev(geom)
@@ -318,8 +337,9 @@ def test_focus_host_matches_oracle(ctx):
 @pytest.mark.parametrize("Np", [1024, 3000, 4100])
 def test_focus_host_pulse_groups_match_oracle(ctx, Np):
     """Host-buffer focusing uploads pulse groups while earlier groups are
-    compressed and backprojected (2 groups of 1500 pulses at 3000, 4 uneven
-    groups at 4100); the accumulated image meets the same bound."""
+    compressed and backprojected (one group at 1024; at 3000 a first group
+    of 375 pulses, then 2 of about 1312; at 4100 a first of 512, then 4 of
+    about 897); the accumulated image meets the same bound."""
     geom, tg, ph = small_inputs(Np=Np, K=64, n_targets=3)
     grid = ImageGrid(40, 36, 0.2, -0.1, 4.0, 3.5)
     img = ctx.focus_host(ph, geom.ant, geom.r0, geom.f0, grid, 256, geom.maxWr)
