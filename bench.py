@@ -32,6 +32,9 @@ import numpy as np  # noqa: E402
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak
 PEAK_TRANS_TOPS = 9.83     # 256 CU x 16 lanes/clk x 2.4 GHz (quarter-rate transcendentals)
+# Non-packed FP32 FMA rate measured by `bench.py --peaks` on MI355X (profiles/gfx950_peaks.md):
+# packed FP32 code generation is disabled, so this is the attainable FP32 ceiling.
+MEASURED_NONPACKED_TFLOPS = 72.68
 FLOPS_PER_PP = 30.0
 TRANS_PER_PP = 3.0
 
@@ -402,6 +405,8 @@ def main() -> int:
         "traffic_source": traffic_src,
         "kernel": "bp_mixed",
         "trans_frac": round(TRANS_PER_PP * pp / (t_bp * 1e-3) / 1e12 / PEAK_TRANS_TOPS, 4),
+        "nonpacked_fp32_measured": MEASURED_NONPACKED_TFLOPS,
+        "frac_of_nonpacked": round(achieved / MEASURED_NONPACKED_TFLOPS, 4),
     }
 
     autofocus = None
