@@ -679,9 +679,14 @@ __global__ __launch_bounds__(kThreads) void kb_rows_lds(PgaBatchDev b, int g0, i
     if (apply) row_apply_lds_body(p, blockIdx.x - off[k]);
     else row_dft_lds_body(p, blockIdx.x - off[k]);
 }
+// M <= 4096: capped at 128 VGPRs for four waves per SIMD (the LDS allows four
+// blocks per CU); the few spills cost less than the lost occupancy (64 uneven
+// images, 5 iterations: 6.10-6.16 -> 5.98-6.01 ms).  The single-image kernel
+// spills more under the same cap and is left uncapped.
 template <int LOG2M>
-__global__ __launch_bounds__(fft::group_threads(LOG2M)) void kb_rows(PgaBatchDev b, int g0, int m,
-                                                                    int apply) {
+__global__ __launch_bounds__(fft::group_threads(LOG2M))
+__attribute__((amdgpu_waves_per_eu(LOG2M <= 12 ? 4 : 1))) void kb_rows(PgaBatchDev b, int g0,
+                                                                        int m, int apply) {
     const int *ids = b.grp + g0, *off = ids + m;
     const int k = owner(off, m, blockIdx.x);
     const PgaImage &p = b.img[ids[k]];
