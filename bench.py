@@ -66,13 +66,37 @@ def make_inputs(ctx, cfg, torch, dev):
 
 
 def host_threads() -> int:
-    """Threads the CPU legs may use: the process's CPU affinity, at most 16
-    (the host share of one GPU)."""
+    """Threads the CPU legs use: the process's CPU affinity, at most 16.  A
+    GPU of this machine comes with a 16-core host share (OMP_NUM_THREADS is
+    16 there), so more threads would time cores another GPU's host work
+    owns."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
         n = os.cpu_count() or 1
     return max(1, min(16, n))
+
+
+def host_info() -> dict:
+    """What the CPU baseline ran on: logical CPUs, the affinity count, the
+    OMP_NUM_THREADS setting and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        aff = None
+    return {"nproc": os.cpu_count(), "affinity": aff,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model,
+            "threads_used": host_threads(),
+            "thread_cap": "16 = the host-core share of one GPU on this machine"}
 
 
 def cpu_baseline(cfg, geom, grid, ph_host, target_s=4.0):
@@ -114,6 +138,7 @@ def cpu_baseline(cfg, geom, grid, ph_host, target_s=4.0):
                    f"{threads} OpenMP threads, median of 3 runs ({dts[1]:.1f} s each)"),
         "single_thread_value": rate1 / 1e9,
         "single_thread_sample": f"first {P1} pulses, 1 thread, median of 3 ({dts1[1]:.2f} s)",
+        "host": host_info(),
     }
 
 

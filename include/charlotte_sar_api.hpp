@@ -131,7 +131,7 @@ int create_SARImageFormationParams(const SAR_Aperture<T> &ap, SAR_ImageFormation
     p.update(ap);
     p.Wx_m = p.max_Wx_m;
     p.Wy_m = p.max_Wy_m;
-    p.N_x_pix = (int)((float)p.Wx_m * p.N_y_pix / p.Wy_m);
+    p.N_x_pix = sarf_detail::square_pixel_columns<T>(p.Wx_m, p.N_y_pix, p.Wy_m);
     return EXIT_SUCCESS;
 }
 

@@ -379,6 +379,21 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *desc,
                const double *f0,   /* host, Np */
                float *img);        /* host, complex Ny x Nx */
 
+/* Matched-filter image formation on host buffers: H2D, sarf_matched_filter, */
+/* D2H.  Synchronous.  This is what focus_SAR_image does when the image      */
+/* parameters select MATCHED_FILTER (run_mf, src/cpuBackProjection/          */
+/* cpuBackProjection.hpp:204-228, dispatched at :255-257); the C++ wrapper  */
+/* in include/uncc_sar_focusing.hpp passes the pulses run_mf uses (index 2  */
+/* on, :218).  With SARF_BP_ACCUMULATE in desc->flags img is read and added */
+/* to; otherwise it is overwritten.                                         */
+int sarf_focus_mf(sarf_ctx *ctx, const sarf_mf_desc *desc,
+                  const float *ph,    /* host, complex K x Np */
+                  const double *ant,  /* host, Np x 3 */
+                  const double *r0,   /* host, Np */
+                  const double *f0,   /* host, Np: frequency of sample 0 */
+                  const double *df,   /* host, Np: frequency step */
+                  float *img);        /* host, complex Ny x Nx */
+
 /* ------------------------------------------------------------------------ */
 /* Point-scatterer phase-history synthesis (test/benchmark data source;     */
 /* data model of matlab/formImagePointTargetBP.m:63-81):                    */
@@ -404,18 +419,27 @@ int sarf_synth_phase_history_nufft(sarf_ctx *ctx, int K, int Np, int S,
 
 /* ------------------------------------------------------------------------ */
 /* 8-bit dB display image: the quantisation of writeBMPFile                 */
-/* (include/uncc_sar_globals.hpp:66-118), in float as the reference does it: */
+/* (include/uncc_sar_globals.hpp:66-118), byte for byte:                    */
 /*   pixelf = (float)(255.0/dyn) * (20 log10(|I| / max|I|) + dyn),          */
 /*   byte = pixelf < 0 ? 0 : (unsigned char)pixelf,                         */
-/* rows top-down, row r showing y = Ny-1-r (far range on top, :85-98):      */
-/* out[r*Nx + x].  log10 is the device library's, so a byte can differ by   */
-/* one level from the host's where pixelf lands within an ulp of an integer.*/
-/* ------------------------------------------------------------------------ */
+/* in float with the host's log10, rows top-down, row r showing y = Ny-1-r  */
+/* (far range on top, :85-98): out[r*Nx + x].  The kernel compares          */
+/* r = |I|/max|I| with the 255 level thresholds of sarf_db8_levels, so no   */
+/* device log10 enters the result.                                          */
 int sarf_image_db8(sarf_ctx *ctx,
                    const void *img,        /* device, complex64 Ny x Nx (img[x*Ny + y]) */
                    int Nx, int Ny, float dyn_range_dB,
                    unsigned char *out,     /* device, Nx*Ny bytes */
                    void *stream);
+
+/* Host only (no device needed): the thresholds sarf_image_db8 quantises   */
+/* with.  levels[b], b = 1..255, is the smallest float r in [0, 1] whose    */
+/* byte above is >= b (+inf when no r <= 1 reaches b); levels[0] = 0.  The  */
+/* byte of r is the number of levels[1..255] <= r.  Found by bisection over */
+/* float bit patterns with the reference's float expression.  No reference  */
+/* counterpart (the reference evaluates log10 per pixel, uncc_sar_globals.hpp:*/
+/* 99-104).                                                                 */
+int sarf_db8_levels(float dyn_range_dB, float *levels /* host, 256 floats */);
 
 #ifdef __cplusplus
 }

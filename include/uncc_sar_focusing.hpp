@@ -11,13 +11,16 @@
 //                                   statistics (:224-393)
 //   SAR_ImageFormationParameters<T> image geometry (:395-475)
 //   focus_SAR_image / cuda_focus_SAR_image
-//                                   range compression + backprojection
-//                                   (:509-512; cuda_sar_focusing.hpp:526-723)
-// Both focusing calls run on the MI355X through sarf_focus (there is no CPU
-// path in this library).  Differences from the reference, all fixes: N_fft
-// > numRangeSamples zero-pads the profiles (a TODO in the reference), dR and
-// the pixel grid are never rounded to FP32 at kilometre ranges, the
-// <double,double> instantiation works, the device is never reset.
+//                                   range compression + backprojection, or
+//                                   the matched filter (:509-512;
+//                                   cpuBackProjection.hpp:230-262;
+//                                   cuda_sar_focusing.hpp:526-723)
+// Both focusing calls run on the MI355X through sarf_focus (backprojection)
+// or sarf_focus_mf (matched filter); there is no CPU path in this library.
+// Differences from the reference, all fixes: N_fft > numRangeSamples
+// zero-pads the profiles (a TODO in the reference), dR and the pixel grid
+// are never rounded to FP32 at kilometre ranges, the <double,double>
+// instantiation works, the device is never reset.
 #ifndef SARF_UNCC_SAR_FOCUSING_HPP
 #define SARF_UNCC_SAR_FOCUSING_HPP
 
@@ -165,6 +168,17 @@ inline T mean_of(const std::vector<T> &v) {
     for (const T &x : v) s += (double)x;
     return (T)(s / (double)v.size());
 }
+// Image width of the default image, in the reference's operation order
+// (uncc_sar_focusing.hpp:461, charlotte_sar_api.hpp:189): the float product
+// Wx * N_y is truncated to int first, then divided by Wy in the precision
+// of T, and the quotient truncated again when it is stored.  Dividing
+// before truncating gives a different width for about 0.3 % of geometries.
+template <typename T>
+inline int square_pixel_columns(T Wx, int Ny, T Wy) {
+    const int wx_ny = (int)((float)Wx * Ny);
+    const T cols = wx_ny / Wy;
+    return (int)cols;
+}
 }  // namespace sarf_detail
 
 // Fills the derived fields of an aperture (uncc_sar_focusing.hpp:224-393):
@@ -310,7 +324,7 @@ class SAR_ImageFormationParameters {
         p.update(ap);
         p.Wx_m = p.max_Wx_m;
         p.Wy_m = p.max_Wy_m;
-        p.N_x_pix = (int)((float)p.Wx_m * p.N_y_pix / p.Wy_m);
+        p.N_x_pix = sarf_detail::square_pixel_columns<MyT>(p.Wx_m, p.N_y_pix, p.Wy_m);
         return p;
     }
 };
@@ -383,21 +397,93 @@ int sarf_focus_aperture(const SAR_Aperture<T> &ap, const SAR_ImageFormationParam
     return SARF_OK;
 }
 
-// GPU range compression + backprojection (cuda_sar_focusing.hpp:526-723).
+// Matched-filter image of `ap` on the pixel grid of `p` (output: N_y_pix x
+// N_x_pix complex, y fastest), on the GPU: what run_mf computes
+// (cpuBackProjection.hpp:165-228) -- every pixel, every sample, no range
+// compression and no swath gate -- over the pulses run_mf visits, index 2
+// to Np-1 (:218).  The frequencies of pulse n are the stepped ramp through
+// freq[n,0] and freq[n,K-1] (both record formats are stepped-frequency:
+// uncc_sar_focusing.hpp:246-308).  Returns SARF_OK or a negative sarf.h code.
+template <typename T, typename P>
+int sarf_mf_aperture(const SAR_Aperture<T> &ap, const SAR_ImageFormationParameters<P> &p,
+                     CArray<T> &out) {
+    sarf_ctx *ctx = sarf_default_context();
+    if (!ctx) return SARF_ERR_NODEVICE;
+    const int K = ap.numRangeSamples, Np = ap.numAzimuthSamples;
+    if (K <= 0 || Np <= 0 || (int)ap.sampleData.data.size() < K * Np ||
+        (int)ap.Ant_x.data.size() < Np || (int)ap.slant_range.data.size() < Np ||
+        (int)ap.freq.data.size() < K * Np || p.N_x_pix < 2 || p.N_y_pix < 2)
+        return SARF_ERR_INVALID;
+    const size_t npix = (size_t)p.N_x_pix * p.N_y_pix;
+    if (out.size() != npix) out.resize(npix);
+    const int n0 = 2;  // run_mf skips pulses 0 and 1
+    const int np = Np - n0;
+    if (np <= 0) {
+        for (size_t i = 0; i < npix; i++) out[i] = Complex<T>(T(0), T(0));
+        return SARF_OK;
+    }
+    std::vector<float> ph(2 * (size_t)K * np);
+    for (size_t i = 0; i < (size_t)K * np; i++) {
+        const auto &v = ap.sampleData.data[(size_t)n0 * K + i];
+        ph[2 * i] = (float)v._M_real;
+        ph[2 * i + 1] = (float)v._M_imag;
+    }
+    std::vector<double> ant(3 * (size_t)np), r0(np), f0(np), df(np);
+    for (int n = 0; n < np; n++) {
+        const int m = n0 + n;
+        ant[3 * n] = ap.Ant_x.data[m];
+        ant[3 * n + 1] = ap.Ant_y.data[m];
+        ant[3 * n + 2] = ap.Ant_z.data[m];
+        r0[n] = ap.slant_range.data[m];
+        f0[n] = ap.freq.data[(size_t)m * K];
+        df[n] = K > 1 ? ((double)ap.freq.data[(size_t)m * K + K - 1] - f0[n]) / (K - 1) : 0.0;
+    }
+    sarf_mf_desc d{};
+    d.Nx = p.N_x_pix;
+    d.Ny = p.N_y_pix;
+    d.K = K;
+    d.Np = np;
+    d.x0 = p.x0_m;
+    d.y0 = p.y0_m;
+    d.Wx = p.Wx_m;
+    d.Wy = p.Wy_m;
+    std::vector<float> img(2 * npix);
+    const int rc = sarf_focus_mf(ctx, &d, ph.data(), ant.data(), r0.data(), f0.data(), df.data(),
+                                 img.data());
+    if (rc != SARF_OK) return rc;
+    for (size_t i = 0; i < npix; i++) out[i] = Complex<T>((T)img[2 * i], (T)img[2 * i + 1]);
+    return SARF_OK;
+}
+
+// Focusing call of the GPU build (cuda_sar_focusing.hpp:526-723): range
+// compression + backprojection, or (which the reference's GPU entry refuses,
+// :537-545) the matched filter of run_mf.
 template <typename T, typename P>
 void cuda_focus_SAR_image(const SAR_Aperture<T> &sar_data,
                           const SAR_ImageFormationParameters<P> &sar_image_params,
                           CArray<T> &output_image) {
-    if (sar_image_params.algorithm != SAR_ImageFormationParameters<P>::BACKPROJECTION) {
-        std::cerr << "cuda_focus_SAR_image: only BACKPROJECTION is available" << std::endl;
-        return;
+    int rc;
+    switch (sar_image_params.algorithm) {
+        case SAR_ImageFormationParameters<P>::BACKPROJECTION:
+            rc = sarf_focus_aperture(sar_data, sar_image_params, output_image);
+            break;
+        case SAR_ImageFormationParameters<P>::MATCHED_FILTER:
+            rc = sarf_mf_aperture(sar_data, sar_image_params, output_image);
+            break;
+        default:
+            std::cerr << "focus_SAR_image: algorithm requested is not available" << std::endl;
+            return;
     }
-    const int rc = sarf_focus_aperture(sar_data, sar_image_params, output_image);
-    if (rc != SARF_OK) std::cerr << "cuda_focus_SAR_image: " << sarf_last_error() << std::endl;
+    if (rc != SARF_OK)
+        std::cerr << "focus_SAR_image: " << (rc == SARF_ERR_INVALID && !*sarf_last_error()
+                                                 ? "aperture or image parameters incomplete"
+                                                 : sarf_last_error())
+                  << std::endl;
 }
 
 // Focusing entry of the reference's CPU build (uncc_sar_focusing.hpp:509-512,
-// cpuBackProjection.hpp:230-262); here it runs on the GPU as well.
+// cpuBackProjection.hpp:230-262): BACKPROJECTION -> run_bp, MATCHED_FILTER ->
+// run_mf (:249-257); here both run on the GPU.
 template <typename T, typename P>
 void focus_SAR_image(const SAR_Aperture<T> &SARData, const SAR_ImageFormationParameters<P> &SARImgParams,
                      CArray<T> &output_image) {

@@ -51,6 +51,7 @@ _SIG = {
     "orc_entropy_f64": (_d, [_P, C.c_long]),
     "orc_bp_adjoint_f64": (_i, [_i, _i, _i, _i, _d, _d, _d, _d, _d, _P, _P, _P, _P, _P, _P]),
     "orc_db8": (_i, [_i, _i, _P, _f, _P]),
+    "orc_db8_level_mismatches": (C.c_long, [_f, _P, C.POINTER(C.c_uint)]),
 }
 
 _LIB = None
@@ -194,6 +195,19 @@ def db8(img, dyn_range_db: float = 70.0) -> np.ndarray:
     if lib().orc_db8(Nx, Ny, _p(img), dyn_range_db, _p(out)) != 0:
         raise ValueError("bad db8 arguments")
     return out
+
+
+def db8_level_mismatches(dyn_range_db: float, levels) -> tuple[int, int]:
+    """Exhaustive check of a 256-entry level table against the reference's
+    dB quantisation of r = |I|/max over every float r in [0, 1]: returns
+    (number of mismatching r, smallest mismatching bit pattern or
+    0xffffffff)."""
+    lv = np.ascontiguousarray(levels, dtype=np.float32)
+    if lv.shape != (256,):
+        raise ValueError("levels must hold 256 floats")
+    first = C.c_uint(0)
+    bad = lib().orc_db8_level_mismatches(dyn_range_db, _p(lv), C.byref(first))
+    return int(bad), int(first.value)
 
 
 def reference_trajectory_f64(params, D: int, Np: int) -> np.ndarray:

@@ -580,3 +580,41 @@ int orc_db8(int Nx, int Ny, const float *img, float dyn, unsigned char *out) {
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Exhaustive check of a level table for the dB quantisation above: for     */
+/* every float r in [0, 1] (bit patterns 0 .. 0x3f800000), the byte of the   */
+/* reference expression (uncc_sar_globals.hpp:99-104, as orc_db8 evaluates  */
+/* it for r = |I|/max) must equal the number of levels[1..255] <= r.        */
+/* Returns the number of mismatching r; *first_bad gets the smallest        */
+/* mismatching bit pattern (0xffffffff if none).                            */
+/* ------------------------------------------------------------------------ */
+static unsigned char db8_byte_of_ratio(float r, float dyn, float scale) {
+    float l = log10f(r);
+    float t = 20.0f * l;
+    float u = t + dyn;
+    float v = scale * u;
+    return (v < 0.0f || v != v) ? 0 : (unsigned char)v;
+}
+
+long orc_db8_level_mismatches(float dyn, const float *levels, unsigned int *first_bad) {
+    const float scale = (float)(255.0 / (double)dyn);
+    const unsigned int top = 0x3f800000u;
+    long bad = 0;
+    unsigned int first = 0xffffffffu;
+#pragma omp parallel for schedule(static, 1 << 20) reduction(+ : bad) reduction(min : first)
+    for (long u = 0; u <= (long)top; u++) {
+        unsigned int bits = (unsigned int)u;
+        float r;
+        memcpy(&r, &bits, 4);
+        int lo = 0;
+        for (int step = 128; step > 0; step >>= 1)
+            if (lo + step <= 255 && levels[lo + step] <= r) lo += step;
+        if (lo != (int)db8_byte_of_ratio(r, dyn, scale)) {
+            bad++;
+            if (bits < first) first = bits;
+        }
+    }
+    *first_bad = first;
+    return bad;
+}

@@ -46,15 +46,24 @@ def _run(cmd: list[str], verbose: bool) -> None:
 
 
 def build_library(verbose: bool = True) -> str:
-    """Compile every HIP source under csrc/ into libsarf.so (always rebuilds)."""
+    """Compile every HIP/C++ source under csrc/ (one hipcc per source, in
+    parallel) and link them into libsarf.so (always rebuilds)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     if not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip"))) + sorted(
         glob.glob(os.path.join(CSRC, "*.cpp"))
     )
+    obj_dir = os.path.join(PKG_DIR, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
+    objs = [os.path.join(obj_dir, os.path.basename(s) + ".o") for s in srcs]
+    jobs = max(1, min(8, len(srcs), os.cpu_count() or 1))
+    with ThreadPoolExecutor(jobs) as pool:
+        list(pool.map(lambda so: _run([HIPCC, *HIP_FLAGS, "-c", f"-I{INCLUDE}", f"-I{CSRC}",
+                                       "-o", so[1], so[0]], verbose), zip(srcs, objs)))
     tmp = LIB_PATH + ".tmp"
-    cmd = [HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}", "-o", tmp, *srcs, "-lz"]
-    _run(cmd, verbose)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"], verbose)
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

@@ -71,6 +71,7 @@ SIGNATURES = {
     "sarf_range_compress_scenes": (_i, [_vp, _i, C.c_void_p, _vp]),
     "sarf_backproject": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _vp, _vp]),
     "sarf_focus": (_i, [_vp, C.POINTER(FocusDesc), _vp, _vp, _vp, _vp, _vp]),
+    "sarf_focus_mf": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sarf_synth_phase_history": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp]),
     "sarf_image_entropy": (_i, [_vp, _vp, C.c_long, _i, _vp, _vp]),
     "sarf_bp_batch_entropy": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _i, _i, _vp, _vp,
@@ -85,6 +86,7 @@ SIGNATURES = {
     "sarf_tune_table_set": (_i, [_vp, C.c_void_p, _i]),
     "sarf_matched_filter": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sarf_image_db8": (_i, [_vp, _vp, _i, _i, C.c_float, _vp, _vp]),
+    "sarf_db8_levels": (_i, [C.c_float, _vp]),
     "sarf_autotune_scenes": (_i, [_vp, _i, C.c_void_p, _i, _vp, _vp, _vp]),
     "sarf_pga_scenes": (_i, [_vp, _i, C.c_void_p, _i, _d, _vp]),
 }

@@ -356,4 +356,32 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const d
     return SARF_OK;
 }
 
+int sarf_focus_mf(sarf_ctx *ctx, const sarf_mf_desc *d, const float *ph, const double *ant,
+                  const double *r0, const double *f0, const double *df, float *img) {
+    clear_error();
+    SARF_TRY(use_device(ctx));
+    if (!d || !ph || !ant || !r0 || !f0 || !df || !img)
+        return fail(SARF_ERR_INVALID, "sarf_focus_mf: null argument");
+    if (d->K < 1 || d->Np < 1 || d->Nx < 2 || d->Ny < 2)
+        return fail(SARF_ERR_INVALID, "sarf_focus_mf: bad sizes K=%d Np=%d Nx=%d Ny=%d", d->K,
+                    d->Np, d->Nx, d->Ny);
+    const size_t ph_bytes = sizeof(float2) * (size_t)d->K * d->Np;
+    const size_t img_bytes = sizeof(float2) * (size_t)d->Nx * d->Ny;
+    void *dph, *dimg, *dgeo;
+    SARF_TRY(scratch_get(ctx, "focus.ph", ph_bytes, &dph));
+    SARF_TRY(scratch_get(ctx, "focus.img", img_bytes, &dimg));
+    SARF_TRY(scratch_get(ctx, "focus.geo", sizeof(double) * 6 * d->Np, &dgeo));
+    double *dant = (double *)dgeo, *dr0 = dant + 3 * d->Np, *df0 = dr0 + d->Np, *ddf = df0 + d->Np;
+    SARF_HIP(hipMemcpy(dph, ph, ph_bytes, hipMemcpyHostToDevice));
+    SARF_HIP(hipMemcpy(dant, ant, sizeof(double) * 3 * d->Np, hipMemcpyHostToDevice));
+    SARF_HIP(hipMemcpy(dr0, r0, sizeof(double) * d->Np, hipMemcpyHostToDevice));
+    SARF_HIP(hipMemcpy(df0, f0, sizeof(double) * d->Np, hipMemcpyHostToDevice));
+    SARF_HIP(hipMemcpy(ddf, df, sizeof(double) * d->Np, hipMemcpyHostToDevice));
+    sarf_mf_desc md = *d;
+    if (md.flags & SARF_BP_ACCUMULATE) SARF_HIP(hipMemcpy(dimg, img, img_bytes, hipMemcpyHostToDevice));
+    SARF_TRY(sarf_matched_filter(ctx, &md, dph, dant, dr0, df0, ddf, dimg, nullptr));
+    SARF_HIP(hipMemcpy(img, dimg, img_bytes, hipMemcpyDeviceToHost));
+    return SARF_OK;
+}
+
 }  // extern "C"
