@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define SARF_VERSION 100
+#define SARF_VERSION 101
 
 /* status codes */
 #define SARF_OK 0
@@ -68,9 +68,24 @@ const char *sarf_last_error(void);
  * Replaces initialize_GPUMATLAB / initialize_CUDAResources
  * (src/gpuBackProjection/cuda_sar_focusing/cuda_sar_focusing.hpp:403-478),
  * without the per-call cudaDeviceReset of finalize_CUDAResources (:482-519).
- * A context may be used by one host thread at a time. */
+ * A context may be used by one host thread at a time.  Its calls may go to
+ * different streams: a call on a stream other than the previous call's waits
+ * (on the device) for that call, since calls share the context's scratch;
+ * sarf_focus / sarf_focus_mf wait on the host.  Not while a stream is being
+ * captured: a caller that captures work of one context on several streams
+ * orders them itself.  Scratch that has to grow is kept, not freed, until
+ * sarf_ctx_trim or sarf_ctx_destroy, so work enqueued earlier (or a captured
+ * graph) never refers to freed memory. */
 int sarf_ctx_create(int device, sarf_ctx **out);
 int sarf_ctx_destroy(sarf_ctx *ctx);
+
+/* Waits for the device, then frees what the context keeps only for work    */
+/* enqueued earlier: scratch buffers replaced by larger ones and the device */
+/* state of every scene / image list uploaded by sarf_range_compress_scenes,*/
+/* sarf_bp_scenes and sarf_pga_scenes.  HIP graphs captured before the call */
+/* must not be replayed afterwards; the next batched call uploads again.     */
+/* No reference counterpart.                                                 */
+int sarf_ctx_trim(sarf_ctx *ctx);
 
 /* ------------------------------------------------------------------------ */
 /* Range compression: prof[n] = fftshift(ifft(ph[n] zero-padded to Nfft))   */
@@ -83,7 +98,10 @@ int sarf_ctx_destroy(sarf_ctx *ctx);
 typedef struct {
     int K;     /* frequency samples per pulse, 1 <= K <= Nfft */
     int Np;    /* pulses, >= 1 */
-    int Nfft;  /* power of two, 32 <= Nfft <= 16384 (FP64: <= 65536) */
+    int Nfft;  /* any length >= K: powers of two up to 2^24 (up to 16384 in one LDS
+                  pass, longer through global-memory passes); other lengths up to 8192
+                  by Bluestein's identity, longer ones (and FP64) by a direct DFT
+                  (O(K) per bin, at most 2^20) */
     int flags; /* SARF_RC_* */
 } sarf_rc_desc;
 
@@ -98,11 +116,13 @@ int sarf_range_compress(sarf_ctx *ctx, const sarf_rc_desc *desc,
 /* Range compression of many independent scenes (FP32), each as one         */
 /* sarf_range_compress call: the scenes of one Nfft share one persistent    */
 /* launch over all their pulses.  Results equal per-scene calls bit for bit.*/
-/* The scene list is uploaded (and the call waits for the copy) only when   */
-/* it differs from the previous call on this context, so a repeated call    */
-/* can be captured into a HIP graph (a new list during capture returns      */
-/* SARF_ERR_INVALID).  The reference compresses one aperture per call       */
-/* (cuda_sar_focusing.hpp:616-621).                                         */
+/* Each distinct scene list is uploaded once (the call waits for the copy)  */
+/* into device memory of its own, kept until sarf_ctx_trim or destroy; a    */
+/* list seen before only enqueues kernels, so after one call outside a      */
+/* stream capture the same call can be captured into a HIP graph, and that  */
+/* graph stays valid whatever other lists later calls use (a new list       */
+/* during capture returns SARF_ERR_INVALID).  The reference compresses one  */
+/* aperture per call (cuda_sar_focusing.hpp:616-621).                       */
 typedef struct {
     const void *ph;       /* device float2 K x Np */
     void *prof;           /* device float2 Nfft x Np */
@@ -268,6 +288,92 @@ int sarf_bp_batch_entropy(sarf_ctx *ctx, const sarf_bp_desc *desc,
                           void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Multi-resolution grid search: the level loop of the reference's autofocus*/
+/* driver (src/autofocus/autofocus_gpu.cu:886-948 for the linear model,     */
+/* :1036-1102 for the quadratic one; cuGridSearch's search_by_value_* and   */
+/* find_extrema).  Each level evaluates every point of the box              */
+/*   p_d = start_d + i_d (end_d - start_d) / (num_d - 1), i_d < num_d        */
+/* (an axis with num_d = 1 stays at start_d; first axis slowest) with one    */
+/* sarf_bp_batch_entropy launch, takes the first minimum, then shrinks the  */
+/* box around it: start'_d = m_d - (end_d - start_d)/4 and, with            */
+/* SARF_GRID_SHRINK_REFERENCE, end'_d = m_d + (end_d - start'_d)/4 -- the    */
+/* reference's update order, which reads the already-moved start (:938-939)*/
+/* -- or, with SARF_GRID_SHRINK_CENTRED, end'_d = m_d + (end_d - start_d)/4. */
+/* Models: SARF_GRID_MODEL_COEF -- p is the coefficient vector of           */
+/* sarf_bp_batch_entropy's linear model (dims = nbasis; basis and dir as    */
+/* there, bp->flags may hold SARF_BATCH_DIR_PER_BASIS); SARF_GRID_MODEL_REF_D7 */
+/* / _D10 -- p is the reference's parameter vector of absolute trajectories */
+/* (gridSearchErrorFunctions_dft.cuh:279-291: x_i = p1 i p6 + p0, ... for D */
+/* = 7; x_i = p2 i^2 + p1 i p9 + p0, ... for D = 10), ant, basis and dir are */
+/* ignored.  Synchronous: best (host, dims), *best_entropy, levels[l] (host,*/
+/* one per level, may be NULL) and cov (host, dims x dims, may be NULL) are */
+/* filled on return.  cov is the covariance estimate of the last level's    */
+/* minimiser (the reference prints cuGridSearch's covariance_by_value, a    */
+/* library that is not available; this estimate is defined at              */
+/* sarf_grid_covariance).                                                   */
+/* ------------------------------------------------------------------------ */
+#define SARF_GRID_MAX_DIMS 10
+#define SARF_GRID_MODEL_COEF 0
+#define SARF_GRID_MODEL_REF_D7 7
+#define SARF_GRID_MODEL_REF_D10 10
+#define SARF_GRID_SHRINK_REFERENCE 0
+#define SARF_GRID_SHRINK_CENTRED 1
+
+typedef struct {
+    int dims;     /* parameters per candidate, 1..SARF_GRID_MAX_DIMS */
+    int model;    /* SARF_GRID_MODEL_* */
+    int levels;   /* resolution levels (the reference's multiRes), 1..64 */
+    int shrink;   /* SARF_GRID_SHRINK_* */
+    double start[SARF_GRID_MAX_DIMS];
+    double end[SARF_GRID_MAX_DIMS];
+    int num[SARF_GRID_MAX_DIMS];  /* samples per axis, 1..4096; product <= 2^21 */
+} sarf_grid_desc;
+
+typedef struct {
+    double start[SARF_GRID_MAX_DIMS]; /* the box this level searched */
+    double end[SARF_GRID_MAX_DIMS];
+    double best[SARF_GRID_MAX_DIMS];  /* its minimiser */
+    double entropy;                   /* entropy at the minimiser */
+    float gpu_ms;                     /* device time of the level's launch */
+    int ncand;                        /* candidates evaluated */
+} sarf_grid_level;
+
+int sarf_grid_search(sarf_ctx *ctx, const sarf_bp_desc *bp,
+                     const void *prof,     /* device float2 Nfft x Np */
+                     const double *ant,    /* device Np x 3 (COEF model) or NULL */
+                     const double *r0, const double *f0,
+                     const sarf_grid_desc *grid,
+                     const double *basis,  /* device (COEF model) or NULL */
+                     const double *dir,    /* device (COEF model) or NULL */
+                     double *best,         /* host, dims */
+                     double *best_entropy, /* host, 1, may be NULL */
+                     double *cov,          /* host, dims x dims, may be NULL */
+                     sarf_grid_level *levels, /* host, grid->levels, may be NULL */
+                     void *stream);
+
+/* Host only: the covariance estimate of sarf_grid_search for objective     */
+/* values[c] over the box of `grid` (start/end/num, candidates ordered as   */
+/* above) with minimiser index `best`: a Laplace approximation -- a full    */
+/* quadratic f = c + g.u + u^T H u / 2 fitted by least squares to the 3^m   */
+/* values around the minimiser (window moved inward at the box edge) over   */
+/* the m axes with >= 3 samples, u in parameter units, and cov = H^-1 there */
+/* when H is positive definite.  Axes of one sample: zero rows and columns; */
+/* axes of two samples or a fit that is not positive definite: NaN.  No     */
+/* reference counterpart (cuGridSearch's covariance_by_value is absent).    */
+int sarf_grid_covariance(const sarf_grid_desc *grid, const double *values, long best,
+                         double *cov /* host, dims x dims */);
+
+/* Host only: the reference's trajectory fits that centre its grid          */
+/* (bestFit / quadFit, src/autofocus/autofocus_gpu.cu:315-416), over every  */
+/* skip-th sample of values (e.g. Ant_x) against the pulse index i:         */
+/* least squares with double sums of the float products values[i] * i (and  */
+/* * i again), results rounded to float.  coeffs = {intercept, slope} and   */
+/* {constant, linear, quadratic}.  Index powers are exact here; the         */
+/* reference forms i^4 in int, which wraps from i = 216.                    */
+int sarf_fit_linear(const float *values, int n, int skip, float *coeffs /* 2 */);
+int sarf_fit_quadratic(const float *values, int n, int skip, float *coeffs /* 3 */);
+
+/* ------------------------------------------------------------------------ */
 /* Adjoint backprojection: grad[n] = dE/da_n (FP64, Np x 3), the gradient    */
 /* of the image entropy with respect to each antenna phase centre (R0 held   */
 /* fixed), given the forward image img = sarf_backproject(...) of the same   */
@@ -290,11 +396,12 @@ int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *desc,
 /* through an atomic counter).  Each scene is described as for              */
 /* sarf_backproject, with its own device buffers; images are overwritten.   */
 /* tile: 0 = chosen for the finest-spaced scene, else 1..5 or 11..14.      */
-/* The work list is uploaded (and the call waits for the copy) only when    */
-/* the descriptors or tile differ from the previous call on this context;   */
-/* a repeated call only enqueues kernels, so after one call outside a       */
-/* stream capture the same call can be captured into a HIP graph (a new     */
-/* list during capture returns SARF_ERR_INVALID).                           */
+/* Each distinct work list (descriptors and tile) is uploaded once (the     */
+/* call waits for the copy) into device memory of its own, kept until       */
+/* sarf_ctx_trim or destroy; a list seen before only enqueues kernels, so   */
+/* after one call outside a stream capture the same call can be captured   */
+/* into a HIP graph, which stays valid whatever lists later calls use (a    */
+/* new list during capture returns SARF_ERR_INVALID).                       */
 /* The reference focuses one aperture per call (cuda_focus_SAR_image,       */
 /* cuda_sar_focusing.hpp:526-723).                                          */
 /* ------------------------------------------------------------------------ */
@@ -324,15 +431,21 @@ int sarf_autotune_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scenes, i
 /* ------------------------------------------------------------------------ */
 /* Phase gradient autofocus of a focused image, in place (matlab/PGA.m:     */
 /* 151-232; C++ restatement src/autofocus/cpu/autofocus_cpu.cpp:249-474).   */
-/* Rows are range (y), columns azimuth (x); img is img[x*Ny + y].  Runs a    */
-/* fixed number of iterations (no early stop).  af_ph (device, Nx doubles)  */
-/* receives the accumulated detrended phase estimate, rms (device, iters)   */
-/* the rms of each iteration's estimate.                                    */
+/* Rows are range (y), columns azimuth (x); img is img[x*Ny + y].  Runs     */
+/* `iters` iterations, or with rms_stop > 0 stops at the first iteration    */
+/* whose detrended estimate has rms < rms_stop, without applying that       */
+/* estimate -- the stopping rule of the C++ restatement                     */
+/* (autofocus_cpu.cpp:432, threshold 0.1 there); everything stays on the    */
+/* device, no host synchronisation.  af_ph (device, Nx doubles) receives    */
+/* the accumulated detrended phase of the applied iterations, rms (device,  */
+/* iters) the rms of each iteration's estimate (NaN for iterations that did */
+/* not run when rms_stop > 0).                                               */
 /* ------------------------------------------------------------------------ */
 typedef struct {
     int Nx, Ny;      /* image size; 2 <= Nx <= 4096 */
-    int iters;       /* iterations */
+    int iters;       /* iterations (the maximum when rms_stop > 0) */
     double db_down;  /* window cut-off below the peak column energy, dB (30) */
+    double rms_stop; /* > 0: early-stop threshold (rad); 0: fixed count */
 } sarf_pga_desc;
 
 int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *desc,
@@ -344,11 +457,13 @@ int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *desc,
 /* PGA on many images of different sizes at once: every step of sarf_pga  */
 /* is one launch over all images (the row transforms one launch per       */
 /* transform length), with results identical to sarf_pga on each image.  */
-/* The images' device state is uploaded (and the call waits for the copy)  */
-/* only when the scene list differs from the previous call on this        */
-/* context, so after one call outside a stream capture the same call can  */
-/* be captured into a HIP graph.  No reference counterpart: the reference  */
-/* autofocuses one image per call (autofocus_cpu.cpp:249-474).            */
+/* The images' device state (descriptors and work arrays) lives in device  */
+/* memory of its own per distinct list (and iters, rms_stop), uploaded once */
+/* (the call waits for the copy) and kept until sarf_ctx_trim or destroy,   */
+/* so after one call outside a stream capture the same call can be captured*/
+/* into a HIP graph that later calls with other lists do not disturb.  No   */
+/* reference counterpart: the reference autofocuses one image per call      */
+/* (autofocus_cpu.cpp:249-474).                                             */
 typedef struct {
     void *img;      /* device float2 Ny x Nx, updated in place */
     double *af_ph;  /* device, Nx */
@@ -357,7 +472,8 @@ typedef struct {
 } sarf_pga_scene;
 
 int sarf_pga_scenes(sarf_ctx *ctx, int nscenes, const sarf_pga_scene *scenes, int iters,
-                    double db_down, void *stream);
+                    double db_down, double rms_stop /* as in sarf_pga_desc, per image */,
+                    void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Whole focusing call on host buffers: H2D, range compression,             */

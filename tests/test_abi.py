@@ -47,7 +47,7 @@ def test_binding_covers_every_declared_symbol():
 @pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="libsarf.so not built")
 def test_version_and_error_string_without_device():
     lib = _lib.load()
-    assert lib.sarf_version() == 100
+    assert lib.sarf_version() == 101
     assert lib.sarf_last_error() is not None
 
 

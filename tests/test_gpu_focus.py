@@ -67,18 +67,43 @@ def test_rc_f64_matches_oracle(ctx, nfft):
     assert oracle.rel_l2(prof, oracle.range_compress_f64(ph, nfft)) < 1e-12
 
 
-@pytest.mark.parametrize("nfft", [37, 424, 1000, 3000])
+@pytest.mark.parametrize("nfft", [3, 37, 424, 1000, 3000, 8191, 8192 + 57])
 def test_rc_non_power_of_two_matches_oracle(ctx, nfft):
-    """Direct-DFT path for Nfft that is not a power of two (N_fft = K is the
-    reference's default).  FP32 <= 2e-6, FP64 <= 1e-12 relative L2."""
+    """Nfft that is not a power of two (N_fft = K is the reference's default,
+    charlotte_sar_api.hpp:173): Bluestein's identity up to 8192 (odd lengths
+    included: the fftshift remap), a direct DFT above; FP32 <= 2e-6 relative
+    L2 against the FP64 oracle on the same complex64 input, with taper and
+    ramp on one case; FP64 (direct) <= 1e-12."""
     rng = np.random.default_rng(nfft)
-    K = min(nfft, 424)
-    ph = rng.standard_normal((4, K)) + 1j * rng.standard_normal((4, K))
-    ref = oracle.range_compress_f64(ph, nfft)
-    p32 = ctx.range_compress(to_dev(ph.astype(np.complex64)), nfft).cpu().numpy()
+    Np = 5
+    for K in sorted({1, min(nfft, 424), nfft}):
+        ph = rng.standard_normal((Np, K)) + 1j * rng.standard_normal((Np, K))
+        p32 = ctx.range_compress(to_dev(ph.astype(np.complex64)), nfft).cpu().numpy()
+        e = oracle.rel_l2(p32, oracle.range_compress_f64(ph.astype(np.complex64), nfft))
+        assert e < 2e-6, (nfft, K, e)
     p64 = ctx.range_compress(to_dev(ph), nfft).cpu().numpy()
-    assert oracle.rel_l2(p32, oracle.range_compress_f64(ph.astype(np.complex64), nfft)) < 2e-6
-    assert oracle.rel_l2(p64, ref) < 1e-12
+    assert oracle.rel_l2(p64, oracle.range_compress_f64(ph, nfft)) < 1e-12
+    taper = np.hanning(K).astype(np.float32)
+    f0 = 9.6e9 + 1e6 * np.arange(Np)
+    r0 = 12500.0 + 0.37 * np.arange(Np)
+    ph32 = ph.astype(np.complex64)
+    pt = ctx.range_compress(to_dev(ph32), nfft, taper=to_dev(taper),
+                            ramp=(to_dev(f0), to_dev(r0))).cpu().numpy()
+    ref = oracle.range_compress_f64(ph32, nfft, taper=taper, ramp=(f0, r0))
+    assert oracle.rel_l2(pt, ref) < 2e-6
+
+
+@pytest.mark.parametrize("nfft", [2, 4, 16, 32768, 65536, 1 << 18])
+def test_rc_power_of_two_outside_lds_range(ctx, nfft):
+    """Powers of two below 32 and above 16384 (global-memory Stockham passes
+    in FP32): <= 2e-6 relative L2 against the FP64 oracle."""
+    rng = np.random.default_rng(nfft)
+    Np = 3
+    for K in sorted({1, max(1, nfft // 3), nfft}):
+        ph = (rng.standard_normal((Np, K)) + 1j * rng.standard_normal((Np, K))).astype(np.complex64)
+        prof = ctx.range_compress(to_dev(ph), nfft).cpu().numpy()
+        e = oracle.rel_l2(prof, oracle.range_compress_f64(ph, nfft))
+        assert e < 2e-6, (nfft, K, e)
 
 
 def test_rc_rejects_bad_sizes(ctx):
@@ -87,7 +112,7 @@ def test_rc_rejects_bad_sizes(ctx):
         ctx.range_compress(ph, 8)  # K > Nfft
     assert e.value.code == _lib.SARF_ERR_INVALID
     with pytest.raises(_lib.SarfError) as e:
-        ctx.range_compress(ph, 16)
+        ctx.range_compress(ph, (1 << 20) + 1)  # beyond the direct transform
     assert e.value.code == _lib.SARF_ERR_UNSUPPORTED
 
 
@@ -577,7 +602,9 @@ def test_image_db8_matches_oracle(ctx, Nx, Ny, dyn):
     print(f"db8 {Nx}x{Ny} dyn {dyn}: {np.count_nonzero(d)} bytes differ, levels used "
           f"{len(np.unique(ref))}")
     assert d.max() == 0
-    assert got.max() == 255
+    # the brightest pixel (r = 1) gets the expression's top byte: 255 at
+    # 70 dB, 254 at 45.5 dB, where (float)(255/dyn) * dyn rounds below 255
+    assert got.max() == ref.max() >= 254
 
 
 @pytest.mark.parametrize("dyn", [70.0, 40.0])

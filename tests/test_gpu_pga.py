@@ -78,3 +78,41 @@ def test_pga_scenes_equal_per_image_calls(ctx):
             assert torch.equal(a2, af_ph), rep
             assert torch.equal(r2, rms), rep
 
+
+
+@pytest.mark.parametrize("nx,ny", [(64, 64), (600, 96)])
+def test_pga_rms_stop_ends_before_applying(ctx, nx, ny):
+    """rms_stop (the C++ restatement's `if (rms < 0.1) break`,
+    autofocus_cpu.cpp:432): with the threshold set just above the fixed-count
+    run's rms at iteration 3, the run stops at the first iteration k whose
+    rms is below it -- rms[0..k] equal the fixed run's, later entries are
+    NaN, and the image and af_ph equal a fixed run of k iterations (the
+    estimate of iteration k is not applied).  Single and batched entries."""
+    _, bad = _degraded_image(nx, ny, seed=11)
+    iters = 8
+    t = torch.from_numpy(bad.copy()).to(DEV)
+    _, rms_fixed = ctx.pga(t, iters)
+    rf = rms_fixed.cpu().numpy()
+    thr = float(rf[3]) * (1 + 1e-9)
+    k = int(np.nonzero(rf < thr)[0][0])
+    assert k <= 3
+    ref = torch.from_numpy(bad.copy()).to(DEV)
+    af_k, rms_k = ctx.pga(ref, k)
+    t = torch.from_numpy(bad.copy()).to(DEV)
+    af_s, rms_s = ctx.pga(t, iters, rms_stop=thr)
+    rs = rms_s.cpu().numpy()
+    np.testing.assert_array_equal(rs[:k + 1], rf[:k + 1])
+    assert np.all(np.isnan(rs[k + 1:]))
+    assert torch.equal(t, ref)
+    assert torch.equal(af_s, af_k)
+    imgs = [torch.from_numpy(bad.copy()).to(DEV), torch.from_numpy(bad.copy()).to(DEV)]
+    res = ctx.pga_scenes(imgs, iters, rms_stop=thr)
+    torch.cuda.synchronize()
+    for img, (a2, r2) in zip(imgs, res):
+        assert torch.equal(img, ref)
+        assert torch.equal(a2, af_k)
+        np.testing.assert_array_equal(r2.cpu().numpy()[:k + 1], rf[:k + 1])
+    # no stop: the fixed-count result, bit for bit
+    t2 = torch.from_numpy(bad.copy()).to(DEV)
+    _, r3 = ctx.pga(t2, iters, rms_stop=1e-12)
+    np.testing.assert_array_equal(r3.cpu().numpy(), rf)

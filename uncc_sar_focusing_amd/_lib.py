@@ -67,6 +67,7 @@ SIGNATURES = {
     "sarf_last_error": (C.c_char_p, []),
     "sarf_ctx_create": (_i, [_i, C.POINTER(_vp)]),
     "sarf_ctx_destroy": (_i, [_vp]),
+    "sarf_ctx_trim": (_i, [_vp]),
     "sarf_range_compress": (_i, [_vp, C.POINTER(RcDesc), _vp, _vp, _vp, _vp, _vp, _vp]),
     "sarf_range_compress_scenes": (_i, [_vp, _i, C.c_void_p, _vp]),
     "sarf_backproject": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -88,7 +89,12 @@ SIGNATURES = {
     "sarf_image_db8": (_i, [_vp, _vp, _i, _i, C.c_float, _vp, _vp]),
     "sarf_db8_levels": (_i, [C.c_float, _vp]),
     "sarf_autotune_scenes": (_i, [_vp, _i, C.c_void_p, _i, _vp, _vp, _vp]),
-    "sarf_pga_scenes": (_i, [_vp, _i, C.c_void_p, _i, _d, _vp]),
+    "sarf_pga_scenes": (_i, [_vp, _i, C.c_void_p, _i, _d, _d, _vp]),
+    "sarf_grid_search": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                              _vp, _vp, _vp, _vp]),
+    "sarf_grid_covariance": (_i, [_vp, _vp, C.c_long, _vp]),
+    "sarf_fit_linear": (_i, [_vp, _i, _i, _vp]),
+    "sarf_fit_quadratic": (_i, [_vp, _i, _i, _vp]),
 }
 
 
@@ -122,12 +128,33 @@ class SceneDesc(C.Structure):
 
 
 class PgaDesc(C.Structure):
-    _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("iters", C.c_int), ("db_down", C.c_double)]
+    _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("iters", C.c_int), ("db_down", C.c_double),
+                ("rms_stop", C.c_double)]
 
 
 class RcScene(C.Structure):
     _fields_ = [("ph", _vp), ("prof", _vp), ("taper", _vp), ("f0", _vp), ("r0", _vp),
                 ("K", C.c_int), ("Np", C.c_int), ("Nfft", C.c_int), ("flags", C.c_int)]
+
+
+SARF_GRID_MAX_DIMS = 10
+SARF_GRID_MODEL_COEF = 0
+SARF_GRID_MODEL_REF_D7 = 7
+SARF_GRID_MODEL_REF_D10 = 10
+SARF_GRID_SHRINK_REFERENCE = 0
+SARF_GRID_SHRINK_CENTRED = 1
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("dims", C.c_int), ("model", C.c_int), ("levels", C.c_int), ("shrink", C.c_int),
+                ("start", C.c_double * SARF_GRID_MAX_DIMS), ("end", C.c_double * SARF_GRID_MAX_DIMS),
+                ("num", C.c_int * SARF_GRID_MAX_DIMS)]
+
+
+class GridLevel(C.Structure):
+    _fields_ = [("start", C.c_double * SARF_GRID_MAX_DIMS), ("end", C.c_double * SARF_GRID_MAX_DIMS),
+                ("best", C.c_double * SARF_GRID_MAX_DIMS), ("entropy", C.c_double),
+                ("gpu_ms", C.c_float), ("ncand", C.c_int)]
 
 
 class PgaScene(C.Structure):

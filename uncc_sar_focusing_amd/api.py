@@ -283,6 +283,61 @@ class Context:
                                              self._stream()))
         return out
 
+    def grid_search(self, prof: torch.Tensor, r0: torch.Tensor, f0: torch.Tensor,
+                    grid: ImageGrid, maxWr: float, start, end, num, levels: int, *,
+                    model: str = "coef", ant: torch.Tensor | None = None,
+                    basis: torch.Tensor | None = None, direction: torch.Tensor | None = None,
+                    shrink: str = "reference", tile: int = 0,
+                    los: tuple[float, float] | None = None):
+        """Multi-resolution grid search (sarf_grid_search).  model "coef":
+        parameters are batch_entropy coefficients (needs ant, basis,
+        direction); "ref7" / "ref10": the reference's absolute trajectory
+        parameters (ant/basis/direction unused).  Returns (best (dims,),
+        best_entropy, covariance (dims, dims), per-level dicts)."""
+        import numpy as np
+
+        Np, nfft = prof.shape
+        _need(prof, torch.complex64, (Np, nfft), "prof")
+        _need(r0, torch.float64, (Np,), "r0")
+        _need(f0, torch.float64, (Np,), "f0")
+        start = np.asarray(start, dtype=np.float64)
+        end = np.asarray(end, dtype=np.float64)
+        num = np.asarray(num, dtype=np.int64)
+        D = start.size
+        if not (end.size == D and num.size == D and 1 <= D <= _lib.SARF_GRID_MAX_DIMS):
+            raise ValueError("start, end and num must have the same length 1..10")
+        models = {"coef": _lib.SARF_GRID_MODEL_COEF, "ref7": _lib.SARF_GRID_MODEL_REF_D7,
+                  "ref10": _lib.SARF_GRID_MODEL_REF_D10}
+        shrinks = {"reference": _lib.SARF_GRID_SHRINK_REFERENCE,
+                   "centred": _lib.SARF_GRID_SHRINK_CENTRED}
+        flags = 0
+        if models[model] == _lib.SARF_GRID_MODEL_COEF:
+            if ant is None or basis is None or direction is None:
+                raise ValueError("model 'coef' needs ant, basis and direction")
+            _need(ant, torch.float64, (Np, 3), "ant")
+            _need(basis, torch.float64, (D, Np), "basis")
+            per_basis = direction.dim() == 3
+            _need(direction, torch.float64, (D, Np, 3) if per_basis else (Np, 3), "direction")
+            flags = _lib.SARF_BATCH_DIR_PER_BASIS if per_basis else 0
+        g = _lib.GridDesc()
+        g.dims, g.model, g.levels, g.shrink = D, models[model], levels, shrinks[shrink]
+        for d in range(D):
+            g.start[d], g.end[d], g.num[d] = float(start[d]), float(end[d]), int(num[d])
+        bd = BpDesc(grid.Nx, grid.Ny, nfft, Np, grid.x0, grid.y0, grid.Wx, grid.Wy, maxWr,
+                    _lib.SARF_BP_MIXED, flags, tile, 0, *(los if los is not None else (0.0, 0.0)))
+        best = np.zeros(D)
+        best_e = C.c_double(0.0)
+        cov = np.zeros((D, D))
+        lv = (_lib.GridLevel * levels)()
+        check(self.lib.sarf_grid_search(self._h, C.byref(bd), _ptr(prof), _ptr(ant), _ptr(r0),
+                                        _ptr(f0), C.byref(g), _ptr(basis), _ptr(direction),
+                                        best.ctypes.data_as(_P), C.byref(best_e),
+                                        cov.ctypes.data_as(_P), lv, self._stream()))
+        out = [dict(start=np.array(L.start[:D]), end=np.array(L.end[:D]),
+                    best=np.array(L.best[:D]), entropy=float(L.entropy), gpu_ms=float(L.gpu_ms),
+                    ncand=int(L.ncand)) for L in lv]
+        return best, float(best_e.value), cov, out
+
     def entropy_grad(self, prof: torch.Tensor, ant: torch.Tensor, r0: torch.Tensor,
                      f0: torch.Tensor, grid: ImageGrid, maxWr: float, img: torch.Tensor,
                      ent: torch.Tensor, *, tile: int = 0, los: tuple[float, float] | None = None,
@@ -345,20 +400,23 @@ class Context:
         return descs, keep
 
     # ----------------------------------------------------------------- PGA
-    def pga(self, img: torch.Tensor, iters: int, db_down: float = 30.0):
+    def pga(self, img: torch.Tensor, iters: int, db_down: float = 30.0, rms_stop: float = 0.0):
         """Phase gradient autofocus of img (Nx, Ny) complex64 in place
-        (PGA.m:151-232).  Returns (af_ph (Nx,) float64, rms (iters,) float64)
-        device tensors."""
+        (PGA.m:151-232).  rms_stop > 0 ends at the first iteration whose
+        estimate's rms is below it, unapplied (autofocus_cpu.cpp:432; the
+        later rms entries are NaN).  Returns (af_ph (Nx,) float64, rms
+        (iters,) float64) device tensors."""
         Nx, Ny = img.shape
         _need(img, torch.complex64, (Nx, Ny), "img")
         af_ph = torch.empty(Nx, dtype=torch.float64, device=img.device)
         rms = torch.empty(max(iters, 1), dtype=torch.float64, device=img.device)
-        d = _lib.PgaDesc(Nx, Ny, iters, db_down)
+        d = _lib.PgaDesc(Nx, Ny, iters, db_down, rms_stop)
         check(self.lib.sarf_pga(self._h, C.byref(d), _ptr(img), _ptr(af_ph), _ptr(rms),
                                 self._stream()))
         return af_ph, rms[:iters]
 
-    def pga_scenes(self, imgs, iters: int, db_down: float = 30.0, *, out=None):
+    def pga_scenes(self, imgs, iters: int, db_down: float = 30.0, *, out=None,
+                   rms_stop: float = 0.0):
         """PGA on many images (each (Nx, Ny) complex64, sizes may differ) in
         place, one launch per step for all of them (sarf_pga_scenes).  Returns
         a list of (af_ph, rms) device tensors.  `out` may pass the list of
@@ -379,7 +437,8 @@ class Context:
                 rms = torch.empty(max(iters, 1), dtype=torch.float64, device=img.device)
             descs[i] = _lib.PgaScene(img.data_ptr(), af_ph.data_ptr(), rms.data_ptr(), Nx, Ny)
             res.append((af_ph, rms))
-        check(self.lib.sarf_pga_scenes(self._h, n, descs, iters, db_down, self._stream()))
+        check(self.lib.sarf_pga_scenes(self._h, n, descs, iters, db_down, rms_stop,
+                                       self._stream()))
         return [(a, r[:iters]) for a, r in res]
 
     # --------------------------------------------------------------- focus

@@ -1,12 +1,15 @@
 """Autofocus drivers on top of libsarf: batched grid search over trajectory
-candidates (all candidate images in one persistent launch, scored by image
-entropy), with the trajectory model of a line-of-sight (LOS) polynomial.
+candidates (all candidate images of a level in one persistent launch, scored
+by image entropy), multi-resolution refinement with a covariance estimate,
+the trajectory fits that centre the reference's grid, and the
+line-of-sight (LOS) polynomial trajectory model of the batched workload.
 
 Reference counterparts: the cuGridSearch-driven search of
-src/autofocus/autofocus_gpu.cu:707-1260 (candidate trajectory model
-gridSearchErrorFunctions_dft.cuh:279-291, multi-resolution shrink
-autofocus_gpu.cu:938-939) -- here the model is the LOS polynomial of the
-batched workload and the objective the entropy E = -sum p ln p.
+src/autofocus/autofocus_gpu.cu:707-1260 -- level loop with the box shrunk
+around each level's argmin (:886-948, :1036-1102; sarf_grid_search), the
+bestFit / quadFit track fits (:315-416; sarf_fit_linear / sarf_fit_quadratic),
+the candidate trajectory model gridSearchErrorFunctions_dft.cuh:279-291 --
+with the objective the entropy E = -sum p ln p.
 """
 from __future__ import annotations
 
@@ -111,3 +114,115 @@ def perturbed(geom: Geometry, coef, powers=(2, 3, 4), x0: float = 0.0, y0: float
     d = los_directions(geom.ant, x0, y0)
     return Geometry(ant=apply_los_offsets(geom.ant, coef, basis, d), r0=geom.r0.copy(),
                     f0=geom.f0.copy(), df=geom.df, K=geom.K)
+
+
+def detrended_basis(basis: np.ndarray) -> np.ndarray:
+    """Each row of `basis` (nb, Np) minus its least-squares line a + b t
+    over t = linspace(-1, 1, Np).  A LOS offset that is constant or linear in
+    slow time only shifts the image (range and cross-range); with these
+    bases a candidate changes the focus and never the image registration,
+    so candidates are compared on the same scene content."""
+    basis = np.asarray(basis, dtype=np.float64)
+    t = np.linspace(-1.0, 1.0, basis.shape[1])
+    A = np.stack([np.ones_like(t), t], axis=1)
+    coef, *_ = np.linalg.lstsq(A, basis.T, rcond=None)
+    return basis - (A @ coef).T
+
+
+def fit_linear(values, skip: int = 1) -> np.ndarray:
+    """The reference's bestFit (autofocus_gpu.cu:315-371) of a recorded
+    antenna track: float32 [intercept, slope] against the pulse index."""
+    from . import _lib
+
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    out = np.zeros(2, dtype=np.float32)
+    _lib.check(_lib.lib().sarf_fit_linear(v.ctypes.data, v.size, skip, out.ctypes.data))
+    return out
+
+
+def fit_quadratic(values, skip: int = 1) -> np.ndarray:
+    """The reference's quadFit (autofocus_gpu.cu:373-416): float32
+    [constant, linear, quadratic] against the pulse index."""
+    from . import _lib
+
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    out = np.zeros(3, dtype=np.float32)
+    _lib.check(_lib.lib().sarf_fit_quadratic(v.ctypes.data, v.size, skip, out.ctypes.data))
+    return out
+
+
+def grid_covariance(start, end, num, values, best: int) -> np.ndarray:
+    """The covariance estimate sarf_grid_search reports, for objective
+    `values` (one per box point, first axis slowest) with minimiser index
+    `best` (host only)."""
+    from . import _lib
+
+    start = np.asarray(start, dtype=np.float64)
+    D = start.size
+    g = _lib.GridDesc()
+    g.dims, g.model, g.levels, g.shrink = D, _lib.SARF_GRID_MODEL_COEF, 1, 0
+    for d in range(D):
+        g.start[d], g.end[d], g.num[d] = float(start[d]), float(end[d]), int(num[d])
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    cov = np.zeros((D, D))
+    import ctypes as C
+
+    _lib.check(_lib.lib().sarf_grid_covariance(C.byref(g), v.ctypes.data, int(best),
+                                               cov.ctypes.data))
+    return cov
+
+
+def reference_grid_box(ant: np.ndarray, D: int, skip: int = 1):
+    """The first-level box of the reference's driver around the fitted
+    antenna track (autofocus_gpu.cu:842-881 for D = 7: slopes +-2.6 with 23
+    samples, velocity scale 0.9..1.1 with 5; :1000-1029 for D = 10: slopes
+    +-1.3 and quadratic terms +-0.65 with 5 samples each, scale 0.9..1.1
+    with 5; intercepts fixed at the fit).  Returns (start, end, num)."""
+    ant = np.asarray(ant, dtype=np.float64)
+    Np = ant.shape[0]
+    if D == 7:
+        diff, n = 2.6, 23
+        fits = [fit_linear(ant[:, a].astype(np.float32), skip) for a in range(3)]
+        start, end, num = [], [], []
+        for c in fits:
+            start += [float(c[0]), float(c[1]) - diff]
+            end += [float(c[0]), float(c[1]) + diff]
+            num += [1, n]
+    elif D == 10:
+        diff, n = 1.3, 5
+        fits = [fit_quadratic(ant[:, a].astype(np.float32), skip) for a in range(3)]
+        start, end, num = [], [], []
+        for c in fits:
+            start += [float(c[0]), float(c[1]) - diff, float(c[2]) - diff / 2]
+            end += [float(c[0]), float(c[1]) + diff, float(c[2]) + diff / 2]
+            num += [1, n, n]
+    else:
+        raise ValueError("D must be 7 or 10")
+    del Np
+    start.append(0.9)
+    end.append(1.1)
+    num.append(5)
+    return np.array(start), np.array(end), np.array(num)
+
+
+@dataclasses.dataclass
+class MultiResResult:
+    best: np.ndarray          # minimiser of the last level
+    best_entropy: float
+    covariance: np.ndarray    # estimate at the last level (sarf_grid_covariance)
+    levels: list              # per level: start, end, best, entropy, gpu_ms, ncand
+
+    @property
+    def gpu_ms(self) -> float:
+        return float(sum(L["gpu_ms"] for L in self.levels))
+
+
+def grid_search_multires(ctx, prof, r0, f0, grid, maxWr, start, end, num, levels: int, *,
+                         model: str = "coef", ant=None, basis=None, direction=None,
+                         shrink: str = "reference", los=None, tile: int = 0) -> MultiResResult:
+    """Multi-resolution grid search (sarf_grid_search): `levels` boxes, each
+    shrunk to half its span around the previous level's minimiser."""
+    best, e, cov, lv = ctx.grid_search(prof, r0, f0, grid, maxWr, start, end, num, levels,
+                                       model=model, ant=ant, basis=basis, direction=direction,
+                                       shrink=shrink, tile=tile, los=los)
+    return MultiResResult(best=best, best_entropy=e, covariance=cov, levels=lv)

@@ -58,6 +58,30 @@ def entropy(ant: torch.Tensor, op: BpOperator) -> torch.Tensor:
     return BackprojectEntropy.apply(ant, op)
 
 
+def smoothing_operator(Np: int, sigma: float, device) -> "callable":
+    """p = S q: a Gaussian smoothing of `sigma` pulses along slow time (rows
+    of S normalised, so constants pass unchanged), as a differentiable map of
+    (Np, k) parameter tensors.  Descending over q instead of the per-pulse
+    offsets p keeps every update inside the band of a smooth trajectory
+    error; the unknowns stay one per pulse and axis."""
+    i = torch.arange(Np, dtype=torch.float64, device=device)
+    S = torch.exp(-0.5 * ((i[:, None] - i[None, :]) / sigma) ** 2)
+    S = S / S.sum(dim=1, keepdim=True)
+    return lambda q: S @ q
+
+
+def los_residual_rms(err: np.ndarray, direction: np.ndarray) -> float:
+    """rms of the line-of-sight component err_n . dir_n of per-pulse
+    position errors (Np, 3) after removing its least-squares line over
+    normalised slow time: a constant or linear LOS offset only shifts the
+    image, so it is not observable by a focus metric."""
+    r = np.sum(np.asarray(err) * np.asarray(direction), axis=1)
+    t = np.linspace(-1.0, 1.0, r.size)
+    A = np.stack([np.ones_like(t), t], axis=1)
+    c, *_ = np.linalg.lstsq(A, r, rcond=None)
+    return float(np.sqrt(np.mean((r - A @ c) ** 2)))
+
+
 @dataclasses.dataclass
 class GradientResult:
     params: torch.Tensor

@@ -277,6 +277,7 @@ extern "C" int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *d, const 
                                     void *stream) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    StreamOrder order_(ctx, (hipStream_t)stream);
     if (!d || !prof || !ant || !r0 || !f0 || !img || !ent || !grad)
         return fail(SARF_ERR_INVALID, "sarf_bp_entropy_grad: null argument");
     if (d->precision != SARF_BP_MIXED)

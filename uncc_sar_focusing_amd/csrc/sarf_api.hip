@@ -51,9 +51,10 @@ int scratch_get(sarf_ctx *ctx, const char *name, size_t bytes, void **out) {
     Scratch &s = ctx->scratch[name];
     if (s.bytes < bytes) {
         if (s.ptr) {
-            // A previous call may still use the old buffer on some stream.
-            SARF_HIP(hipDeviceSynchronize());
-            SARF_HIP(hipFree(s.ptr));
+            // Work enqueued earlier (on any stream, or in a captured graph)
+            // may still use the old buffer: keep it until sarf_ctx_trim or
+            // sarf_ctx_destroy instead of freeing it here.
+            ctx->retired.push_back(s.ptr);
             s.ptr = nullptr;
             s.bytes = 0;
         }
@@ -67,6 +68,61 @@ int scratch_get(sarf_ctx *ctx, const char *name, size_t bytes, void **out) {
     }
     *out = s.ptr;
     return SARF_OK;
+}
+
+int list_buffer(sarf_ctx *ctx, char tag, const std::vector<unsigned char> &key, size_t bytes,
+                hipStream_t s, const char *who, void **buf, bool *fresh) {
+    std::vector<unsigned char> k;
+    k.reserve(key.size() + 1);
+    k.push_back((unsigned char)tag);
+    k.insert(k.end(), key.begin(), key.end());
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->lists.find(k);
+    if (it != ctx->lists.end() && it->second.bytes >= bytes) {
+        *buf = it->second.ptr;
+        *fresh = false;
+        return SARF_OK;
+    }
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    SARF_HIP(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone)
+        return fail(SARF_ERR_INVALID, "%s: a new list cannot be uploaded during stream capture; "
+                    "make the same call once before capturing", who);
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess)
+        return fail(SARF_ERR_NOMEM, "%s: list state of %zu bytes: %s", who, bytes,
+                    hipGetErrorString(e));
+    if (it != ctx->lists.end()) {
+        ctx->retired.push_back(it->second.ptr);
+        it->second = Scratch{p, bytes};
+    } else {
+        ctx->lists.emplace(std::move(k), Scratch{p, bytes});
+    }
+    *buf = p;
+    *fresh = true;
+    return SARF_OK;
+}
+
+static bool capturing(hipStream_t s) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone;
+}
+
+StreamOrder::StreamOrder(sarf_ctx *c, hipStream_t st) : ctx(c), s(st), active(false) {
+    if (!ctx || !ctx->last_ev || capturing(s)) return;
+    active = true;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->last_valid && ctx->last_stream != s) (void)hipStreamWaitEvent(s, ctx->last_ev, 0);
+}
+
+StreamOrder::~StreamOrder() {
+    if (!active) return;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (hipEventRecord(ctx->last_ev, s) == hipSuccess) {
+        ctx->last_stream = s;
+        ctx->last_valid = true;
+    }
 }
 
 template <typename T2>
@@ -160,10 +216,33 @@ int sarf_ctx_create(int device, sarf_ctx **out) {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(SARF_ERR_NODEVICE, "device %d is %s; libsarf is built for gfx950 only",
                     device, prop.gcnArchName);
+    SARF_HIP(hipSetDevice(device));
+    hipEvent_t ev = nullptr;
+    SARF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     sarf_ctx *ctx = new sarf_ctx();
     ctx->device = device;
     ctx->cu_count = prop.multiProcessorCount;
+    ctx->last_ev = ev;
     *out = ctx;
+    return SARF_OK;
+}
+
+// Frees what the context keeps only so that earlier work stays valid: the
+// retired scratch buffers and the device state of every uploaded list.
+static void trim(sarf_ctx *ctx) {
+    for (void *p : ctx->retired) (void)hipFree(p);
+    ctx->retired.clear();
+    for (auto &kv : ctx->lists)
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    ctx->lists.clear();
+}
+
+int sarf_ctx_trim(sarf_ctx *ctx) {
+    clear_error();
+    SARF_TRY(use_device(ctx));
+    SARF_HIP(hipDeviceSynchronize());
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    trim(ctx);
     return SARF_OK;
 }
 
@@ -174,6 +253,9 @@ int sarf_ctx_destroy(sarf_ctx *ctx) {
     (void)hipDeviceSynchronize();
     for (auto &kv : ctx->tw32) (void)hipFree(kv.second);
     for (auto &kv : ctx->tw64) (void)hipFree(kv.second);
+    for (auto &kv : ctx->rc_blue) (void)hipFree(kv.second);
+    trim(ctx);
+    if (ctx->last_ev) (void)hipEventDestroy(ctx->last_ev);
     for (auto &kv : ctx->scratch)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
@@ -187,6 +269,7 @@ int sarf_range_compress(sarf_ctx *ctx, const sarf_rc_desc *desc, const void *ph,
                         void *stream) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    StreamOrder order_(ctx, (hipStream_t)stream);
     if (!desc || !ph || !prof) return fail(SARF_ERR_INVALID, "sarf_range_compress: null argument");
     return launch_range_compress(ctx, desc, ph, taper, f0, r0, prof, (hipStream_t)stream);
 }
@@ -196,6 +279,7 @@ int sarf_backproject(sarf_ctx *ctx, const sarf_bp_desc *desc, const void *prof,
                      void *stream) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    StreamOrder order_(ctx, (hipStream_t)stream);
     if (!desc || !prof || !ant || !r0 || !f0 || !img)
         return fail(SARF_ERR_INVALID, "sarf_backproject: null argument");
     return launch_backproject(ctx, desc, prof, ant, r0, f0, img, (hipStream_t)stream);
@@ -205,6 +289,12 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const d
                const double *r0, const double *f0, float *img) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    {
+        // the context's scratch may still be in use by its last call on a
+        // caller's stream: wait for that call before reusing it
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (ctx->last_valid) SARF_HIP(hipEventSynchronize(ctx->last_ev));
+    }
     if (!d || !ph || !ant || !r0 || !f0 || !img)
         return fail(SARF_ERR_INVALID, "sarf_focus: null argument");
     if (d->K <= 0 || d->Np <= 0 || d->Nx < 2 || d->Ny < 2 || d->Nfft < d->K)
@@ -360,6 +450,12 @@ int sarf_focus_mf(sarf_ctx *ctx, const sarf_mf_desc *d, const float *ph, const d
                   const double *r0, const double *f0, const double *df, float *img) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    {
+        // the context's scratch may still be in use by its last call on a
+        // caller's stream: wait for that call before reusing it
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (ctx->last_valid) SARF_HIP(hipEventSynchronize(ctx->last_ev));
+    }
     if (!d || !ph || !ant || !r0 || !f0 || !df || !img)
         return fail(SARF_ERR_INVALID, "sarf_focus_mf: null argument");
     if (d->K < 1 || d->Np < 1 || d->Nx < 2 || d->Ny < 2)

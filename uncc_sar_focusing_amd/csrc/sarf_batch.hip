@@ -142,6 +142,7 @@ extern "C" int sarf_bp_batch_entropy(sarf_ctx *ctx, const sarf_bp_desc *d, const
                                      void *stream) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    StreamOrder order_(ctx, (hipStream_t)stream);
     if (!d || !prof || !ant || !r0 || !f0 || !entropy || (nbasis > 0 && (!coef || !basis || !dir)))
         return fail(SARF_ERR_INVALID, "sarf_bp_batch_entropy: null argument");
     if (ncand < 1 || nbasis < 0 || nbasis > 16)
@@ -194,6 +195,7 @@ extern "C" int sarf_image_entropy(sarf_ctx *ctx, const void *img, long npix, int
                                   void *stream) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    StreamOrder order_(ctx, (hipStream_t)stream);
     if (!img || !out || npix < 1) return fail(SARF_ERR_INVALID, "sarf_image_entropy: bad argument");
     const int nblocks = (int)std::min<long>((npix + 255) / 256, 1024);
     void *buf = nullptr;

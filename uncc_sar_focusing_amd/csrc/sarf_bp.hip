@@ -290,9 +290,12 @@ int default_splits(const sarf_ctx *ctx, int ntiles, int Np) {
 
 // Mixed-precision launch with a resolved tile id (> 0) and split count
 // (0 = default rule).
+// ext_part (with room for ext_bytes) replaces the context's partial-image
+// scratch when given.
 int launch_mixed_shape(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, int splits,
                        const void *prof, const double *ant, const double *r0, const double *f0,
-                       void *img, hipStream_t s) {
+                       void *img, hipStream_t s, float2 *ext_part = nullptr,
+                       size_t ext_bytes = 0) {
     BpArgs a{};
     SARF_TRY(bp_fill_args(ctx, d, tile, &a));
     int tx = 0, ty = 0;
@@ -306,8 +309,15 @@ int launch_mixed_shape(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, int split
     a.splits = splits;
     a.split_stride = 0;
     if (splits > 1) {
-        void *buf = nullptr;
-        SARF_TRY(scratch_get(ctx, "bp.partials", sizeof(float2) * npix * splits, &buf));
+        void *buf = ext_part;
+        if (ext_part) {
+            if (sizeof(float2) * (size_t)npix * splits > ext_bytes)
+                return fail(SARF_ERR_INVALID, "backproject: %d pulse splits of a %dx%d image need "
+                            "%zu bytes of partial images, more than the %zu available", splits,
+                            d->Nx, d->Ny, sizeof(float2) * (size_t)npix * splits, ext_bytes);
+        } else {
+            SARF_TRY(scratch_get(ctx, "bp.partials", sizeof(float2) * npix * splits, &buf));
+        }
         part = (float2 *)buf;
         o = part;
         a.split_stride = npix;
@@ -349,8 +359,12 @@ int launch_reduce_partials(const float2 *part, long stride, int splits, long n, 
     return SARF_OK;
 }
 
-int launch_backproject(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof, const double *ant,
-                       const double *r0, const double *f0, void *img, hipStream_t s) {
+// The backprojection of sarf_backproject; with ext_part, partial images go
+// to ext_part (ext_bytes) instead of context scratch, and an automatic or
+// tuned split count is lowered until they fit.
+int launch_backproject_ext(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof,
+                           const double *ant, const double *r0, const double *f0, void *img,
+                           hipStream_t s, float2 *ext_part, size_t ext_bytes) {
     if (d->precision != SARF_BP_MIXED && d->precision != SARF_BP_F64)
         return fail(SARF_ERR_INVALID, "backproject: unknown precision %d", d->precision);
     if (d->precision == SARF_BP_F64) {
@@ -378,7 +392,22 @@ int launch_backproject(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof, c
                         d->Ny, d->Wx, d->Wy);
         tile = bp_pick_tile(d, d->Wx / (d->Nx - 1), d->Wy / (d->Ny - 1));
     }
-    return launch_mixed_shape(ctx, d, tile, splits, prof, ant, r0, f0, img, s);
+    if (ext_part && d->pulse_split <= 0) {
+        int tx = 0, ty = 0;
+        if (!bp_tile_shape(tile, &tx, &ty))
+            return fail(SARF_ERR_INVALID, "backproject: unknown tile id %d", tile);
+        const int ntiles = ((d->Nx + tx - 1) / tx) * ((d->Ny + ty - 1) / ty);
+        if (splits <= 0) splits = default_splits(ctx, ntiles, d->Np);
+        const size_t img_bytes = sizeof(float2) * (size_t)d->Nx * d->Ny;
+        while (splits > 1 && img_bytes * splits > ext_bytes) splits /= 2;
+    }
+    return launch_mixed_shape(ctx, d, tile, splits, prof, ant, r0, f0, img, s, ext_part,
+                              ext_bytes);
+}
+
+int launch_backproject(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof, const double *ant,
+                       const double *r0, const double *f0, void *img, hipStream_t s) {
+    return launch_backproject_ext(ctx, d, prof, ant, r0, f0, img, s, nullptr, 0);
 }
 
 }  // namespace sarf
@@ -410,6 +439,7 @@ extern "C" int sarf_autotune(sarf_ctx *ctx, const sarf_bp_desc *d, const void *p
                              int reps, sarf_tune_result *out, void *stream) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    StreamOrder order_(ctx, (hipStream_t)stream);
     if (!d || !prof || !ant || !r0 || !f0 || !img || !out)
         return fail(SARF_ERR_INVALID, "sarf_autotune: null argument");
     if (d->precision != SARF_BP_MIXED)
@@ -515,3 +545,4 @@ extern "C" int sarf_tune_table_set(sarf_ctx *ctx, const sarf_tune_entry *entries
     for (int i = 0; i < n; i++) ctx->tuned[tune_key(entries[i])] = entries[i];
     return SARF_OK;
 }
+

@@ -18,6 +18,8 @@ namespace sarf {
 
 constexpr double kClight = 299792458.0;   // uncc_sar_focusing.hpp:34
 constexpr double kPi = 3.141592653589793; // uncc_sar_focusing.hpp:36
+// Candidates per level of sarf_grid_search.
+constexpr long kGridMaxCandidates = 1L << 21;
 
 // Records a printf-style message as the calling thread's last error and
 // returns `code`, so call sites read `return fail(SARF_ERR_INVALID, ...)`.
@@ -57,25 +59,30 @@ struct sarf_ctx {
     // DFT twiddle tables e^{+j 2 pi i / N}, keyed by N
     std::map<long, float2 *> tw32;
     std::map<long, double2 *> tw64;
-    // named scratch buffers (partial images, reductions, staging)
+    // Bluestein range-compression tables, keyed by the transform length
+    std::map<long, float2 *> rc_blue;
+    // named scratch buffers (partial images, reductions, staging); a buffer
+    // that has to grow is retired, not freed, so work enqueued earlier --
+    // on any stream, or in a captured graph -- never sees freed memory
     std::map<std::string, sarf::Scratch> scratch;
+    std::vector<void *> retired;
+    // device state of the lists uploaded by the batched entry points, one
+    // allocation per distinct list (entry-point tag + the caller's
+    // descriptors), never overwritten: a graph captured with a list keeps
+    // valid pointers until sarf_ctx_trim or sarf_ctx_destroy
+    std::map<std::vector<unsigned char>, sarf::Scratch> lists;
+    // ordering of this context's calls across streams: the end of the last
+    // call, recorded on its stream
+    hipEvent_t last_ev = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool last_valid = false;
     // backprojection launch shapes chosen by sarf_autotune, keyed by the
     // image geometry: (tile id, pulse splits)
     std::map<std::array<uint64_t, 8>, sarf_tune_entry> tuned;
-    // last scene list uploaded by sarf_bp_scenes (caller's descriptors and
-    // tile), so an identical call re-launches without a copy or a wait
-    std::vector<unsigned char> scenes_key;
-    void *scenes_buf = nullptr;
-    // last scene list uploaded by sarf_range_compress_scenes
-    std::vector<unsigned char> rcs_key;
-    void *rcs_buf = nullptr;
     // host-buffer path: pinned staging (grown on demand) and its copy threads
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
     sarf::HostCopier *copier = nullptr;
-    // last image list uploaded by sarf_pga_scenes (device state bytes)
-    std::vector<unsigned char> pga_key;
-    void *pga_buf = nullptr;
 };
 
 namespace sarf {
@@ -84,6 +91,27 @@ namespace sarf {
 int use_device(sarf_ctx *ctx);
 // Returns a device buffer of at least `bytes` bytes under `name`.
 int scratch_get(sarf_ctx *ctx, const char *name, size_t bytes, void **out);
+// The device buffer dedicated to one list of a batched entry point: `tag`
+// names the entry point, `key` the caller's list (and parameters that shape
+// the device state).  A list seen before returns its buffer with *fresh =
+// false (nothing to upload, nothing to wait for); a new one is allocated
+// (*fresh = true, the caller uploads), which is refused during capture.
+int list_buffer(sarf_ctx *ctx, char tag, const std::vector<unsigned char> &key, size_t bytes,
+                hipStream_t s, const char *who, void **buf, bool *fresh);
+
+// Orders the calls of one context across streams.  The constructor makes
+// stream `s` wait for the end of the context's previous call if that was on
+// another stream; the destructor records the end of this call on `s`.  Both
+// are skipped while `s` is being captured (a graph orders its own nodes).
+struct StreamOrder {
+    sarf_ctx *ctx;
+    hipStream_t s;
+    bool active;
+    StreamOrder(sarf_ctx *c, hipStream_t st);
+    ~StreamOrder();
+    StreamOrder(const StreamOrder &) = delete;
+    StreamOrder &operator=(const StreamOrder &) = delete;
+};
 // Twiddle table for N = 2^log2n (created on first use, synchronously).
 int twiddles_f32(sarf_ctx *ctx, int log2n, const float2 **out);
 int twiddles_f64(sarf_ctx *ctx, int log2n, const double2 **out);
@@ -99,6 +127,9 @@ int launch_range_compress(sarf_ctx *ctx, const sarf_rc_desc *d, const void *ph,
 int launch_backproject(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof,
                        const double *ant, const double *r0, const double *f0, void *img,
                        hipStream_t stream);
+int launch_backproject_ext(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof,
+                           const double *ant, const double *r0, const double *f0, void *img,
+                           hipStream_t stream, float2 *ext_part, size_t ext_bytes);
 // img[i] (+)= sum over s in order of part[s*stride + i], i < n (deterministic).
 int launch_reduce_partials(const float2 *part, long stride, int splits, long n, int accumulate,
                            float2 *img, hipStream_t stream);

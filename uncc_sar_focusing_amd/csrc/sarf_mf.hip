@@ -144,6 +144,7 @@ extern "C" int sarf_matched_filter(sarf_ctx *ctx, const sarf_mf_desc *d, const v
                                    const double *df, void *img, void *stream) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    StreamOrder order_(ctx, (hipStream_t)stream);
     if (!d || !ph || !ant || !r0 || !f0 || !df || !img)
         return fail(SARF_ERR_INVALID, "sarf_matched_filter: null argument");
     if (d->Nx < 2 || d->Ny < 2 || d->K < 1 || d->Np < 1)

@@ -29,11 +29,22 @@ namespace {
 constexpr int kDbThreads = 256;
 constexpr int kDbTile = 64;
 
+// Correctly rounded float quotient and square root: formed in double and
+// rounded to float once.  A double carries 53 bits, more than 2*24 + 2, so
+// rounding the (correctly rounded) double result of a float division or
+// square root to float gives that float operation's correctly rounded
+// result.  (The device's float divide and square root builtins measured one
+// ulp off the host's IEEE results on some |I| / max ratios.)
+__device__ __forceinline__ float div_rn(float a, float b) {
+    return (float)((double)a / (double)b);
+}
+__device__ __forceinline__ float sqrt_rn(float a) { return (float)sqrt((double)a); }
+
 __device__ __forceinline__ float scaled_abs(float2 v) {
     const float s = fmaxf(fabsf(v.x), fabsf(v.y));
     if (s == 0.0f) return s;
-    const float x = __fdiv_rn(v.x, s), y = __fdiv_rn(v.y, s);
-    return __fmul_rn(s, __fsqrt_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y))));
+    const float x = div_rn(v.x, s), y = div_rn(v.y, s);
+    return __fmul_rn(s, sqrt_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y))));
 }
 
 __global__ __launch_bounds__(kDbThreads) void db8_max_partial(const float2 *__restrict__ img,
@@ -89,7 +100,7 @@ __global__ __launch_bounds__(kDbThreads) void db8_quantize(const float2 *__restr
         const int x = x0 + xi;
         unsigned char b = 0;
         if (x < Nx && y < Ny) {
-            const float r = __fdiv_rn(scaled_abs(img[(size_t)x * Ny + y]), m);
+            const float r = div_rn(scaled_abs(img[(size_t)x * Ny + y]), m);
             if (r == r) {  // 0/0 (an all-zero image) quantises to 0
                 // count of thresholds <= r among level[1..255] (non-decreasing)
                 int lo = 0;
@@ -160,6 +171,7 @@ extern "C" int sarf_image_db8(sarf_ctx *ctx, const void *img, int Nx, int Ny, fl
                               unsigned char *out, void *stream) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    StreamOrder order_(ctx, (hipStream_t)stream);
     if (!img || !out) return fail(SARF_ERR_INVALID, "sarf_image_db8: null argument");
     if (Nx < 1 || Ny < 1 || !(dyn_range_dB > 0.0f))
         return fail(SARF_ERR_INVALID, "sarf_image_db8: Nx=%d Ny=%d dyn_range_dB=%g", Nx, Ny,

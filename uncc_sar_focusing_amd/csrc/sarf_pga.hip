@@ -67,18 +67,28 @@ struct PgaImage {
     double2 *ephi;         // Nx
     float *mask;           // Nx
     float2 *H;             // M: FFT of the conjugate chirp
+    float2 *B;             // M: spectrum of this iteration's correction kernel
     float2 *wtab;          // N: chirp w_n = e^{-j pi n^2 / N}
     const float2 *twM, *twN, *ch2n;  // e^{+j 2 pi t/M}, /N, /(2N)
-    int Nx, Ny, L;
+    int *stop;             // 1 once an iteration's rms fell below rms_stop
+    double rms_stop;       // > 0: early stop threshold; 0: fixed iteration count
+    int Nx, Ny, L, iters;
 };
 
+// Early stop: every step after the iteration whose rms fell below the
+// threshold returns at once (uniform per workgroup; kernel boundaries order
+// the flag's write and reads).
+__device__ __forceinline__ bool stopped(const PgaImage &p) {
+    return p.rms_stop > 0.0 && *p.stop != 0;
+}
+
 // In-place radix-2 decimation-in-time FFT of buf[0..M), M = 2^L, by one
-// workgroup of kThreads: forward uses e^{-j 2 pi t/M}, inverse e^{+j 2 pi t/M}
+// workgroup (any size): forward uses e^{-j 2 pi t/M}, inverse e^{+j 2 pi t/M}
 // (unnormalised).  tw[t] = e^{+j 2 pi t / M}, t < M.
 __device__ void lds_fft(float2 *buf, int L, const float2 *__restrict__ tw, bool inverse) {
     const int M = 1 << L;
-    const int tid = threadIdx.x;
-    for (int i = tid; i < M; i += kThreads) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int i = tid; i < M; i += nt) {
         const int r = (int)(__brev((unsigned)i) >> (32 - L));
         if (r > i) {
             const float2 t = buf[i];
@@ -90,7 +100,7 @@ __device__ void lds_fft(float2 *buf, int L, const float2 *__restrict__ tw, bool 
     for (int s = 1; s <= L; s++) {
         const int h = 1 << (s - 1);
         const int tstride = M >> s;
-        for (int b = tid; b < M / 2; b += kThreads) {
+        for (int b = tid; b < M / 2; b += nt) {
             const int j = b & (h - 1);
             const int i0 = ((b >> (s - 1)) << s) + j;
             const int i1 = i0 + h;
@@ -120,7 +130,7 @@ __device__ __forceinline__ void bluestein_core(float2 *buf, int L, const float2 
                                                const float2 *__restrict__ H) {
     const int M = 1 << L;
     lds_fft(buf, L, twM, false);
-    for (int i = threadIdx.x; i < M; i += kThreads) buf[i] = cmul(buf[i], H[i]);
+    for (int i = threadIdx.x; i < M; i += blockDim.x) buf[i] = cmul(buf[i], H[i]);
     __syncthreads();
     lds_fft(buf, L, twM, true);
 }
@@ -133,6 +143,9 @@ __device__ __forceinline__ void bluestein_core(float2 *buf, int L, const float2 
 __device__ void setup_body(const PgaImage &p, bool spectrum) {
     extern __shared__ float2 buf[];
     for (int i = threadIdx.x; i < p.Nx; i += kThreads) p.af_ph[i] = 0.0;
+    if (threadIdx.x == 0) *p.stop = 0;
+    if (p.rms_stop > 0.0)  // iterations that do not run report NaN
+        for (int i = threadIdx.x; i < p.iters; i += kThreads) p.rms[i] = __builtin_nan("");
     if (!spectrum) return;
     const int N = p.Nx, L = p.L, M = 1 << L;
     for (int i = threadIdx.x; i < N; i += kThreads) p.wtab[i] = chirp(i, N, p.ch2n);
@@ -168,6 +181,7 @@ __device__ void transpose_body(const float2 *__restrict__ src, int R, int C,
 // Row y: first maximum of |At[y][x]|^2 over x -> circular shift
 // s_y = ceil(N/2) - idx (mod N).
 __device__ void argmax_body(const PgaImage &p, int y) {
+    if (stopped(p)) return;
     __shared__ float bm[kThreads];
     __shared__ int bi[kThreads];
     const int N = p.Nx, tid = threadIdx.x;
@@ -206,6 +220,7 @@ __device__ void argmax_body(const PgaImage &p, int y) {
 // pa[chunk*N + x] = sum over the chunk's rows of |At[y][x]|^2 (mode 0), or
 // for G (mode 1) num/den parts of phi_dot: Im(conj G_k G_{k+1}), |G_k|^2.
 __device__ void col_partial_body(const PgaImage &p, int mode, int bx, int chunk) {
+    if (stopped(p)) return;
     __shared__ double ra[4][64], rb[4][64];
     const float2 *src = mode == 0 ? p.At : p.G;
     const int N = p.Nx, Ny = p.Ny;
@@ -238,6 +253,7 @@ __device__ void col_partial_body(const PgaImage &p, int mode, int bx, int chunk)
 // Pass 2, block bx: w[x] = sum of the chunk parts (mode 0: energy), or
 // phidot[k] = num / den for k < N-1 (mode 1).
 __device__ void col_final_body(const PgaImage &p, int mode, int bx) {
+    if (stopped(p)) return;
     const int N = p.Nx;
     const int x = bx * kThreads + threadIdx.x;
     if (x >= N) return;
@@ -255,6 +271,7 @@ __device__ void col_final_body(const PgaImage &p, int mode, int bx) {
 // the centre below the cut-off, right = the first column from the centre on
 // below it, minus 1 (parallel max/min reductions).
 __device__ void window_body(const PgaImage &p, double db_down) {
+    if (stopped(p)) return;
     __shared__ double rmax[kIntThreads];
     __shared__ int rl[kIntThreads], rr[kIntThreads];
     const int Nx = p.Nx, tid = threadIdx.x;
@@ -321,6 +338,7 @@ __device__ inline double block_sum_1024(double v, double *red) {
 // correction.  unwrap of a cumulative sum is the cumulative sum of the
 // unwrapped steps, so phi is one block-wide prefix scan of unwrap_step.
 __device__ void integrate_body(const PgaImage &p, int it) {
+    if (stopped(p)) return;
     __shared__ double sc[kIntThreads];
     __shared__ double red[kIntThreads];
     const int Nx = p.Nx, tid = threadIdx.x;
@@ -364,17 +382,29 @@ __device__ void integrate_body(const PgaImage &p, int it) {
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int i = b + j;
+        loc[j] -= slope * i + icpt;
+        if (j < per && i < Nx) r += loc[j] * loc[j];
+    }
+    r = block_sum_1024(r, red);
+    const double rms = sqrt(r / n);
+    if (tid == 0) p.rms[it] = rms;
+    // autofocus_cpu.cpp:432: an estimate below the threshold ends the loop
+    // before it is applied
+    if (p.rms_stop > 0.0 && rms < p.rms_stop) {
+        if (tid == 0) *p.stop = 1;
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int i = b + j;
         if (j < per && i < Nx) {
-            const double v = loc[j] - (slope * i + icpt);
+            const double v = loc[j];
             p.af_ph[i] += v;
             double sn, cs;
             sincos(v, &sn, &cs);
             p.ephi[i] = make_double2(cs, -sn);
-            r += v * v;
         }
     }
-    r = block_sum_1024(r, red);
-    if (tid == 0) p.rms[it] = sqrt(r / n);
 }
 
 // Short rows (M <= 512): one workgroup of kThreads per row, the row in LDS
@@ -384,6 +414,7 @@ __device__ void integrate_body(const PgaImage &p, int it) {
 // G[y][k] = e^{-j 2 pi k (s - fl)/N} DFT_x( mask[(x + s) mod N] At[y][x] )_k,
 // fl = floor(N/2): the shift by s, the mask and ifftshift as one remap.
 __device__ void row_dft_lds_body(const PgaImage &p, int y) {
+    if (stopped(p)) return;
     extern __shared__ float2 buf[];
     const int N = p.Nx, L = p.L, M = 1 << L;
     const int s = p.shift[y];
@@ -413,37 +444,26 @@ __device__ void row_dft_lds_body(const PgaImage &p, int y) {
     }
 }
 
-// Step 6 for row y, in place: At[y] <- IDFT( DFT(At[y]) .* e^{-j phi} ),
-// the inverse as conj(DFT(conj(.)))/N through the same Bluestein core.
+// Step 6 for row y, in place: At[y] <- IDFT_N( DFT_N(At[y]) .* e^{-j phi} ),
+// which is the circular convolution At[y] (*)_N h, h = IDFT_N(e^{-j phi}):
+// the row zero-padded to M, times B = FFT_M(h extended over -(N-1)..N-1)
+// (apply_spectrum_body), back, and the bins N-1 .. 2N-2 are the result.
 __device__ void row_apply_lds_body(const PgaImage &p, int y) {
+    if (stopped(p)) return;
     extern __shared__ float2 buf[];
     const int N = p.Nx, L = p.L, M = 1 << L;
     float2 *row = p.At + (size_t)y * N;
     for (int x = threadIdx.x; x < M; x += kThreads)
-        buf[x] = x < N ? cmul(row[x], p.wtab[x]) : make_float2(0.f, 0.f);
+        buf[x] = x < N ? row[x] : make_float2(0.f, 0.f);
     __syncthreads();
-    bluestein_core(buf, L, p.twM, p.H);
+    lds_fft(buf, L, p.twM, false);
+    for (int k = threadIdx.x; k < M; k += kThreads) buf[k] = cmul(buf[k], p.B[k]);
+    __syncthreads();
+    lds_fft(buf, L, p.twM, true);
     const float invM = 1.0f / (float)M;
-    // Z_k = X_k e^{-j phi_k}; next input conj(Z_k) w_k, zero-padded (each
-    // lane reads and rewrites only its own elements)
-    for (int k = threadIdx.x; k < M; k += kThreads) {
-        float2 v = make_float2(0.f, 0.f);
-        if (k < N) {
-            const float2 w = p.wtab[k];
-            const float2 X = cmul(buf[k], w);
-            const double2 e = p.ephi[k];
-            const float2 Z = cmul(make_float2(X.x * invM, X.y * invM),
-                                  make_float2((float)e.x, (float)e.y));
-            v = cmul(conjf2(Z), w);
-        }
-        buf[k] = v;
-    }
-    __syncthreads();
-    bluestein_core(buf, L, p.twM, p.H);
-    const float scale = invM / (float)N;
     for (int n = threadIdx.x; n < N; n += kThreads) {
-        const float2 u = cmul(buf[n], p.wtab[n]);
-        row[n] = make_float2(u.x * scale, -u.y * scale);
+        const float2 c = buf[n + N - 1];
+        row[n] = make_float2(c.x * invM, c.y * invM);
     }
 }
 
@@ -475,6 +495,7 @@ constexpr int row_lds_elems() {
 // row_dft_lds_body; lds: row_lds_elems<LOG2M>() elements.
 template <int LOG2M>
 __device__ __forceinline__ void row_dft_body(const PgaImage &p, int blk, float2 *lds) {
+    if (stopped(p)) return;
     using S = fft::Shape<LOG2M>;
     constexpr int M = S::N, T = S::T, P = fft::rows_per_group(LOG2M), PAD = fft::padded(LOG2M);
     const int N = p.Nx;
@@ -517,9 +538,12 @@ __device__ __forceinline__ void row_dft_body(const PgaImage &p, int blk, float2 
     }
 }
 
-// Step 6 for the P rows of block blk, in place, as row_apply_lds_body.
+// Step 6 for the P rows of block blk, in place, as row_apply_lds_body: the
+// circular convolution with the correction kernel by two transforms of
+// length M (forward of the zero-padded row, product with B, inverse).
 template <int LOG2M>
 __device__ __forceinline__ void row_apply_body(const PgaImage &p, int blk, float2 *lds) {
+    if (stopped(p)) return;
     using S = fft::Shape<LOG2M>;
     constexpr int M = S::N, T = S::T, P = fft::rows_per_group(LOG2M), PAD = fft::padded(LOG2M);
     const int N = p.Nx;
@@ -534,40 +558,69 @@ __device__ __forceinline__ void row_apply_body(const PgaImage &p, int blk, float
 #pragma unroll
         for (int r = 0; r < S::R0; r++) {
             const int n = j + b * T + r * (M / S::R0);
-            const float2 a = (active && n < N) ? cmul(row[n], p.wtab[n])
-                                                : make_float2(0.f, 0.f);
+            const float2 a = (active && n < N) ? row[n] : make_float2(0.f, 0.f);
             v[b * S::R0 + r] = conjf2(a);
         }
-    bluestein_regs<LOG2M>(v, Lr, j, p.twM, p.H);
-    const float invM = 1.0f / (float)M;
-    // Z_k = X_k e^{-j phi_k}; the next forward transform's input is
-    // conj(Z_k) w_k, zero-padded, fed conjugated in pass-0 order
+    fft::ifft_regs<LOG2M>(v, Lr, j, p.twM);  // v[r] = conj(A_k), k = j + r*T
     float2 u[16];
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int k = j + r * T;
-        float2 nx = make_float2(0.f, 0.f);
-        if (k < N) {
-            const float2 w = p.wtab[k];
-            const float2 X = cmul(v[r], w);
-            const double2 e = p.ephi[k];
-            const float2 Z = cmul(make_float2(X.x * invM, X.y * invM),
-                                  make_float2((float)e.x, (float)e.y));
-            nx = cmul(conjf2(Z), w);
-        }
-        u[r] = conjf2(nx);
-    }
+    for (int r = 0; r < 16; r++) u[r] = cmul(conjf2(v[r]), p.B[j + r * T]);
     fft::to_input_order<LOG2M>(u, v);
-    bluestein_regs<LOG2M>(v, Lr, j, p.twM, p.H);
-    const float scale = invM / (float)N;
+    fft::ifft_regs<LOG2M>(v, Lr, j, p.twM);  // v[r] = M (a * b)_{j + r*T}
+    const float invM = 1.0f / (float)M;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-        const int n = j + r * T;
-        if (active && n < N) {
-            const float2 uo = cmul(v[r], p.wtab[n]);
-            row[n] = make_float2(uo.x * scale, -uo.y * scale);
-        }
+        const int k = j + r * T - (N - 1);
+        if (active && k >= 0 && k < N) row[k] = make_float2(v[r].x * invM, v[r].y * invM);
     }
+}
+
+// After integrate: the correction kernel's spectrum for step 6,
+//   h = IDFT_N(e^{-j phi}) = conj(DFT_N(conj(e^{-j phi}))) / N   (Bluestein),
+//   b_m = h[(m - (N-1)) mod N] for m <= 2N-2, 0 up to M,  B = FFT_M(b),
+// so that IDFT_N(DFT_N(x) .* e^{-j phi}) = (x * b)[N-1 .. 2N-2] for a row x
+// zero-padded to M >= 2N-1.  One workgroup of kThreads per image, dynamic LDS
+// of M float2 (the rotation into b goes through registers, <= M/kThreads
+// values per lane).
+__device__ void apply_spectrum_body(const PgaImage &p) {
+    if (stopped(p)) return;
+    extern __shared__ float2 buf[];
+    constexpr int kMaxPer = (1 << kMaxL) / kThreads;
+    const int N = p.Nx, L = p.L, M = 1 << L;
+    for (int k = threadIdx.x; k < M; k += blockDim.x) {
+        float2 v = make_float2(0.f, 0.f);
+        if (k < N) {
+            const double2 e = p.ephi[k];
+            v = cmul(make_float2((float)e.x, -(float)e.y), p.wtab[k]);
+        }
+        buf[k] = v;
+    }
+    __syncthreads();
+    bluestein_core(buf, L, p.twM, p.H);
+    // h_i = conj(w_i buf[i]) / (M N); b_m = h[(m - (N-1)) mod N], m <= 2N-2
+    const float scale = 1.0f / ((float)M * (float)N);
+    float2 keep[kMaxPer];
+#pragma unroll
+    for (int q = 0; q < kMaxPer; q++) {
+        const int m = threadIdx.x + q * blockDim.x;
+        float2 v = make_float2(0.f, 0.f);
+        if (m < M && m <= 2 * N - 2) {
+            int i = m - (N - 1);
+            if (i < 0) i += N;
+            const float2 X = cmul(buf[i], p.wtab[i]);
+            v = make_float2(X.x * scale, -X.y * scale);
+        }
+        keep[q] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kMaxPer; q++) {
+        const int m = threadIdx.x + q * blockDim.x;
+        if (m < M) buf[m] = keep[q];
+    }
+    __syncthreads();
+    lds_fft(buf, L, p.twM, false);
+    for (int m = threadIdx.x; m < M; m += blockDim.x) p.B[m] = buf[m];
 }
 
 // Blocks of a row step for an image of Ny rows: one workgroup per row up to
@@ -599,6 +652,9 @@ __global__ __launch_bounds__(kIntThreads) void k1_window(PgaImage p, double db_d
 }
 __global__ __launch_bounds__(kIntThreads) void k1_integrate(PgaImage p, int it) {
     integrate_body(p, it);
+}
+__global__ __launch_bounds__(kThreads) void k1_apply_spectrum(PgaImage p) {
+    apply_spectrum_body(p);
 }
 __global__ __launch_bounds__(kThreads) void k1_rows_lds(PgaImage p, int apply) {
     if (apply) row_apply_lds_body(p, blockIdx.x);
@@ -670,6 +726,9 @@ __global__ __launch_bounds__(kIntThreads) void kb_window(PgaBatchDev b, double d
 __global__ __launch_bounds__(kIntThreads) void kb_integrate(PgaBatchDev b, int it) {
     integrate_body(b.img[blockIdx.x], it);
 }
+__global__ __launch_bounds__(kThreads) void kb_apply_spectrum(PgaBatchDev b) {
+    apply_spectrum_body(b.img[blockIdx.x]);
+}
 // Row step of one transform length: grp[g0 .. g0 + m) are the images' ids,
 // grp[g0 + m .. g0 + 2m] their block offsets.
 __global__ __launch_bounds__(kThreads) void kb_rows_lds(PgaBatchDev b, int g0, int m, int apply) {
@@ -710,7 +769,8 @@ size_t small_bytes(int Nx, int Ny, int L) {
     return align256(sizeof(int) * Ny) + 2 * align256(sizeof(double) * kRowChunks * Nx) +
            2 * align256(sizeof(double) * Nx) + align256(sizeof(double2) * Nx) +
            align256(sizeof(float) * Nx) + align256(sizeof(float2) * ((size_t)1 << L)) +
-           align256(sizeof(float2) * Nx);
+           align256(sizeof(float2) * Nx) + align256(sizeof(int)) +
+           align256(sizeof(float2) * ((size_t)1 << L));
 }
 
 void carve(PgaImage &p, char *base) {
@@ -728,6 +788,8 @@ void carve(PgaImage &p, char *base) {
     p.mask = (float *)take(sizeof(float) * p.Nx);
     p.H = (float2 *)take(sizeof(float2) * ((size_t)1 << p.L));
     p.wtab = (float2 *)take(sizeof(float2) * p.Nx);
+    p.stop = (int *)take(sizeof(int));
+    p.B = (float2 *)take(sizeof(float2) * ((size_t)1 << p.L));
 }
 
 int tables(sarf_ctx *ctx, PgaImage &p) {
@@ -796,9 +858,12 @@ extern "C" int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *d, void *img, double
                         double *rms, void *stream) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    StreamOrder order_(ctx, (hipStream_t)stream);
     if (!d || !img || !af_ph || !rms) return fail(SARF_ERR_INVALID, "sarf_pga: null argument");
     SARF_TRY(check_image("sarf_pga", d->Nx, d->Ny));
     if (d->iters < 0) return fail(SARF_ERR_INVALID, "sarf_pga: iters=%d", d->iters);
+    if (!(d->rms_stop >= 0.0))
+        return fail(SARF_ERR_INVALID, "sarf_pga: rms_stop=%g", d->rms_stop);
     hipStream_t s = (hipStream_t)stream;
     PgaImage p{};
     p.Nx = d->Nx;
@@ -807,6 +872,8 @@ extern "C" int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *d, void *img, double
     p.img = (float2 *)img;
     p.af_ph = af_ph;
     p.rms = rms;
+    p.iters = d->iters;
+    p.rms_stop = d->rms_stop;
     const size_t npix = (size_t)p.Nx * p.Ny;
     void *bufA, *bufG, *bufS;
     SARF_TRY(scratch_get(ctx, "pga.At", sizeof(float2) * npix, &bufA));
@@ -832,6 +899,7 @@ extern "C" int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *d, void *img, double
             hipLaunchKernelGGL(k1_col_partial, gcp, dim3(kThreads), 0, s, p, 1);
             hipLaunchKernelGGL(k1_col_final, gcf, dim3(kThreads), 0, s, p, 1);
             hipLaunchKernelGGL(k1_integrate, dim3(1), dim3(kIntThreads), 0, s, p, it);
+            hipLaunchKernelGGL(k1_apply_spectrum, dim3(1), dim3(kThreads), lds, s, p);
             SARF_TRY(k1_row_step(p, 1, s));
             SARF_HIP(hipGetLastError());
         }
@@ -843,11 +911,13 @@ extern "C" int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *d, void *img, double
 }
 
 extern "C" int sarf_pga_scenes(sarf_ctx *ctx, int nscenes, const sarf_pga_scene *scenes, int iters,
-                               double db_down, void *stream) {
+                               double db_down, double rms_stop, void *stream) {
     clear_error();
     SARF_TRY(use_device(ctx));
-    if (nscenes < 1 || !scenes || iters < 0)
-        return fail(SARF_ERR_INVALID, "sarf_pga_scenes: nscenes=%d iters=%d", nscenes, iters);
+    StreamOrder order_(ctx, (hipStream_t)stream);
+    if (nscenes < 1 || !scenes || iters < 0 || !(rms_stop >= 0.0))
+        return fail(SARF_ERR_INVALID, "sarf_pga_scenes: nscenes=%d iters=%d rms_stop=%g", nscenes,
+                    iters, rms_stop);
     const int n = nscenes;
     std::vector<PgaImage> host(n);  // value-initialised: padding is zero (the cache key)
     size_t tot_pix = 0, tot_small = 0;
@@ -863,24 +933,11 @@ extern "C" int sarf_pga_scenes(sarf_ctx *ctx, int nscenes, const sarf_pga_scene 
         p.img = (float2 *)c.img;
         p.af_ph = c.af_ph;
         p.rms = c.rms;
+        p.iters = iters;
+        p.rms_stop = rms_stop;
         tot_pix += align256(sizeof(float2) * (size_t)c.Nx * c.Ny);
         tot_small += small_bytes(p.Nx, p.Ny, p.L);
         SARF_TRY(tables(ctx, p));
-    }
-    void *bufA, *bufG, *bufS;
-    SARF_TRY(scratch_get(ctx, "pgab.At", tot_pix, &bufA));
-    SARF_TRY(scratch_get(ctx, "pgab.G", tot_pix, &bufG));
-    SARF_TRY(scratch_get(ctx, "pgab.small", tot_small, &bufS));
-    {
-        size_t op = 0, os = 0;
-        for (int i = 0; i < n; i++) {
-            PgaImage &p = host[i];
-            p.At = (float2 *)((char *)bufA + op);
-            p.G = (float2 *)((char *)bufG + op);
-            op += align256(sizeof(float2) * (size_t)p.Nx * p.Ny);
-            carve(p, (char *)bufS + os);
-            os += small_bytes(p.Nx, p.Ny, p.L);
-        }
     }
     // block offsets per step kind, and the row-step groups by transform length
     std::vector<int> off((size_t)kOffKinds * (n + 1), 0);
@@ -915,42 +972,46 @@ extern "C" int sarf_pga_scenes(sarf_ctx *ctx, int nscenes, const sarf_pga_scene 
     }
     const size_t b_img = sizeof(PgaImage) * (size_t)n, b_off = sizeof(int) * off.size(),
                  b_grp = sizeof(int) * grp.size();
+    const size_t b_desc = align256(b_img) + align256(b_off) + align256(b_grp);
+    hipStream_t s = (hipStream_t)stream;
+    // Device state of this image list -- descriptors, transposes, spectra and
+    // per-image small state -- in an allocation of its own, keyed by the
+    // caller's list and the parameters stored in the state: a list seen
+    // before launches without copying or waiting, and a graph captured with
+    // it keeps pointers no other list's call writes.
+    std::vector<unsigned char> key(sizeof(sarf_pga_scene) * (size_t)n + sizeof(int) +
+                                   sizeof(double));
+    std::memcpy(key.data(), scenes, sizeof(sarf_pga_scene) * (size_t)n);
+    std::memcpy(key.data() + sizeof(sarf_pga_scene) * (size_t)n, &iters, sizeof(int));
+    std::memcpy(key.data() + sizeof(sarf_pga_scene) * (size_t)n + sizeof(int), &rms_stop,
+                sizeof(double));
     void *buf = nullptr;
-    SARF_TRY(scratch_get(ctx, "pgab.desc", align256(b_img) + align256(b_off) + b_grp, &buf));
+    bool fresh = false;
+    SARF_TRY(list_buffer(ctx, 'p', key, b_desc + 2 * tot_pix + tot_small, s, "sarf_pga_scenes",
+                         &buf, &fresh));
+    char *bufA = (char *)buf + b_desc, *bufG = bufA + tot_pix, *bufS = bufG + tot_pix;
+    {
+        size_t op = 0, os = 0;
+        for (int i = 0; i < n; i++) {
+            PgaImage &p = host[i];
+            p.At = (float2 *)(bufA + op);
+            p.G = (float2 *)(bufG + op);
+            op += align256(sizeof(float2) * (size_t)p.Nx * p.Ny);
+            carve(p, bufS + os);
+            os += small_bytes(p.Nx, p.Ny, p.L);
+        }
+    }
     PgaBatchDev b{};
     b.img = (const PgaImage *)buf;
     b.off = (const int *)((char *)buf + align256(b_img));
     b.grp = (const int *)((char *)buf + align256(b_img) + align256(b_off));
     b.n = n;
-    hipStream_t s = (hipStream_t)stream;
-    // The device state depends only on the caller's scenes and the scratch
-    // buffers: when it matches the last upload into this buffer, launch
-    // again without copying (and without waiting, so the call can be
-    // captured into a graph once the state has been uploaded outside it).
-    std::vector<unsigned char> key(b_img + b_off + b_grp);
-    std::memcpy(key.data(), host.data(), b_img);
-    std::memcpy(key.data() + b_img, off.data(), b_off);
-    std::memcpy(key.data() + b_img + b_off, grp.data(), b_grp);
-    bool cached;
-    {
-        std::lock_guard<std::mutex> lk(ctx->mu);
-        cached = ctx->pga_buf == buf && ctx->pga_key == key;
-    }
-    if (!cached) {
-        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-        SARF_HIP(hipStreamIsCapturing(s, &cap));
-        if (cap != hipStreamCaptureStatusNone)
-            return fail(SARF_ERR_INVALID,
-                        "sarf_pga_scenes: a new scene list cannot be uploaded during stream "
-                        "capture; make the same call once before capturing");
+    if (fresh) {
         // pageable sources: the copies complete before returning
         SARF_HIP(hipMemcpyAsync((void *)b.img, host.data(), b_img, hipMemcpyHostToDevice, s));
         SARF_HIP(hipMemcpyAsync((void *)b.off, off.data(), b_off, hipMemcpyHostToDevice, s));
         SARF_HIP(hipMemcpyAsync((void *)b.grp, grp.data(), b_grp, hipMemcpyHostToDevice, s));
         SARF_HIP(hipStreamSynchronize(s));
-        std::lock_guard<std::mutex> lk(ctx->mu);
-        ctx->pga_buf = buf;
-        ctx->pga_key = std::move(key);
     }
     const auto total = [&](int kind) { return off[(size_t)kind * (n + 1) + n]; };
     const size_t lds = sizeof(float2) * ((size_t)1 << maxL);
@@ -968,6 +1029,7 @@ extern "C" int sarf_pga_scenes(sarf_ctx *ctx, int nscenes, const sarf_pga_scene 
                                b, 1);
             hipLaunchKernelGGL(kb_col_final, dim3(total(kOffColFinal)), dim3(kThreads), 0, s, b, 1);
             hipLaunchKernelGGL(kb_integrate, dim3(n), dim3(kIntThreads), 0, s, b, it);
+            hipLaunchKernelGGL(kb_apply_spectrum, dim3(n), dim3(kThreads), lds, s, b);
             for (const Group &g : groups) SARF_TRY(kb_row_step(b, g.L, g.g0, g.m, g.blocks, 1, s));
             SARF_HIP(hipGetLastError());
         }

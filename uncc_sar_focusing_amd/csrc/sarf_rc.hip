@@ -36,6 +36,9 @@ using fft::dft;
 using fft::lpad;
 using fft::powers16;
 
+// Longest non-power-of-two transform done by Bluestein (M <= 16384 in LDS).
+constexpr int kBluesteinMaxN = 8192;
+
 constexpr int rc_threads(int log2n) {
     return ((1 << log2n) / 16 >= 256) ? (1 << log2n) / 16 : 256;
 }
@@ -241,14 +244,15 @@ __global__ __launch_bounds__(rc_threads(LOG2N)) void rc_lds_f32_jobs(
 }
 
 // ---------------------------------------------------------------------------
-// FP64 path: global-memory Stockham, radix 4 (one radix-2 pass if log2 N is
-// odd).  Pass 0 reads the phase history (zero padding, taper, fftshift
-// fold); the last pass applies 1/N and the ramp.
+// Global-memory Stockham, radix 4 (one radix-2 pass if log2 N is odd): the
+// FP64 path, and FP32 transforms longer than the 16384 bins that fit in
+// LDS.  Pass 0 reads the phase history (zero padding, taper, fftshift fold);
+// the last pass applies 1/N and the ramp.
 // ---------------------------------------------------------------------------
-template <int R>
-__global__ __launch_bounds__(256) void rc_pass_f64(
-    const double2 *__restrict__ in, double2 *__restrict__ out, int N, int Ns, int Np,
-    const double2 *__restrict__ tw, int first, int K, int flags,
+template <typename T2, int R>
+__global__ __launch_bounds__(256) void rc_pass(
+    const T2 *__restrict__ in, T2 *__restrict__ out, int N, int Ns, int Np,
+    const T2 *__restrict__ tw, int first, int K, int flags,
     const float *__restrict__ taper, int last, const double *__restrict__ f0,
     const double *__restrict__ r0) {
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -256,19 +260,23 @@ __global__ __launch_bounds__(256) void rc_pass_f64(
     const long n = gid / per;
     const int j = (int)(gid % per);
     if (n >= Np) return;
-    double2 v[R];
+    using Re = decltype(T2{}.x);
+    T2 v[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int idx = j + r * (N / R);
         if (first) {
-            double2 x = make_double2(0.0, 0.0);
+            T2 x{};
             if (idx < K) {
                 x = in[n * K + idx];
                 if (flags & SARF_RC_TAPER) {
                     x.x *= taper[idx];
                     x.y *= taper[idx];
                 }
-                if (idx & 1) x = make_double2(-x.x, -x.y);
+                if (idx & 1) {
+                    x.x = -x.x;
+                    x.y = -x.y;
+                }
             }
             v[r] = x;
         } else {
@@ -283,13 +291,15 @@ __global__ __launch_bounds__(256) void rc_pass_f64(
     }
     // inverse DFT of size R (R = 2 or 4)
     if constexpr (R == 2) {
-        double2 a = v[0], b = v[1];
+        T2 a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
     } else {
-        double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
-        double2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
-        double2 jb1 = make_double2(-b1.y, b1.x);  // +j * b1
+        T2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        T2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+        T2 jb1;  // +j * b1
+        jb1.x = -b1.y;
+        jb1.y = b1.x;
         v[0] = cadd(a0, b0);
         v[2] = csub(a0, b0);
         v[1] = cadd(a1, jb1);
@@ -307,8 +317,12 @@ __global__ __launch_bounds__(256) void rc_pass_f64(
     const int base = (j - k) * R + k;
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        double2 x = v[r];
-        if (last) x = make_double2(x.x * sr - x.y * si, x.x * si + x.y * sr);
+        T2 x = v[r];
+        if (last) {
+            const Re xr = x.x, xi = x.y;
+            x.x = xr * (Re)sr - xi * (Re)si;
+            x.y = xr * (Re)si + xi * (Re)sr;
+        }
         out[n * N + base + r * Ns] = x;
     }
 }
@@ -369,6 +383,80 @@ __global__ __launch_bounds__(256) void rc_direct(const T2 *__restrict__ ph, int 
         o.x = (decltype(o.x))(ar * (R)sr - ai * (R)si);
         o.y = (decltype(o.y))(ar * (R)si + ai * (R)sr);
         prof[(size_t)n * N + m] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Nfft not a power of two, up to 8192 (e.g. the reference's default
+// N_fft = K, charlotte_sar_api.hpp:173): Bluestein's identity
+// k m = (k^2 + m^2 - (m - k)^2) / 2 turns the length-N inverse DFT into a
+// circular convolution of length M = 2^ceil(log2(2N - 1)):
+//   X_m = c_m sum_k (x_k c_k) conj(c_{m-k}),   c_k = e^{+j pi k^2 / N},
+// formed per pulse by two register-resident radix-16 transforms of length M
+// (sarf_fft.hpp): the forward transform of a_k = x_k c_k as conj(ifft(conj)),
+// the product with C = FFT_M(conj(c)) (a per-N table, circularly extended),
+// and the inverse transform.  c_k comes from the exact k^2 mod 2N in FP64.
+// Zero padding (k >= K) is never loaded; fftshift = cshift((N+1)/2) is an
+// index remap on the store (odd N included), with 1/N and the ramp.  One row
+// of M/16 lanes per pulse, rows packed to >= 256 lanes per workgroup.
+// ---------------------------------------------------------------------------
+template <int LOG2M>
+__global__ __launch_bounds__(fft::group_threads(LOG2M)) void rc_bluestein(
+    const float2 *__restrict__ ph, int K, int N, int Np, int flags,
+    const float *__restrict__ taper, const double *__restrict__ f0,
+    const double *__restrict__ r0, const float2 *__restrict__ chirp,
+    const float2 *__restrict__ C, const float2 *__restrict__ twM, float2 *__restrict__ prof) {
+    using S = fft::Shape<LOG2M>;
+    constexpr int M = S::N, T = S::T, P = fft::rows_per_group(LOG2M);
+    constexpr int PAD = fft::padded(LOG2M);
+    __shared__ float2 lds[P * PAD];
+    const int lp = threadIdx.x / T, j = threadIdx.x % T;
+    const int n = blockIdx.x * P + lp;
+    const bool active = n < Np;
+    float2 *Lr = lds + lp * PAD;
+    const float2 *x = ph + (size_t)(active ? n : 0) * K;
+    float2 v[16];
+#pragma unroll
+    for (int b = 0; b < S::NB0; b++)
+#pragma unroll
+        for (int r = 0; r < S::R0; r++) {
+            const int k = j + b * T + r * (M / S::R0);
+            float2 a = make_float2(0.f, 0.f);
+            if (active && k < K) {
+                float2 xv = x[k];
+                if (flags & SARF_RC_TAPER) {
+                    xv.x *= taper[k];
+                    xv.y *= taper[k];
+                }
+                a = cmul(xv, chirp[k]);
+            }
+            v[b * S::R0 + r] = make_float2(a.x, -a.y);
+        }
+    fft::ifft_regs<LOG2M>(v, Lr, j, twM);  // v[r] = conj(A_k), k = j + r*T
+    float2 u[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) u[r] = cmul(make_float2(v[r].x, -v[r].y), C[j + r * T]);
+    fft::to_input_order<LOG2M>(u, v);
+    fft::ifft_regs<LOG2M>(v, Lr, j, twM);  // v[r] = M (a * conj c)_{j + r*T}
+    if (!active) return;
+    double sr = 1.0 / ((double)M * N), si = 0.0;
+    if (flags & SARF_RC_RAMP) {
+        const double ang = -4.0 * kPi * f0[n] * r0[n] / kClight;
+        sr = cos(ang) / ((double)M * N);
+        si = sin(ang) / ((double)M * N);
+    }
+    const float fr = (float)sr, fi = (float)si;
+    const int half = (N + 1) / 2;
+    float2 *out = prof + (size_t)n * N;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int m = j + r * T;
+        if (m < N) {
+            const float2 X = cmul(v[r], chirp[m]);
+            int o = m - half;
+            if (o < 0) o += N;
+            out[o] = make_float2(X.x * fr - X.y * fi, X.x * fi + X.y * fr);
+        }
     }
 }
 
@@ -440,6 +528,121 @@ int launch_jobs(const sarf_ctx *ctx, int rows, const RcJob *jobs, int njobs, int
     return launch_jobs_rows<LOG2N, 16>(ctx, jobs, njobs, ngroups, tw, s);
 }
 
+// Bluestein tables of a length-N inverse DFT with transform length M = 2^L
+// (cached per N in the context): chirp[k] = e^{+j pi k^2 / N}, k < N, and
+// C = FFT_M(h) with h_d = conj(chirp[d]) for d < N, h_{M-d} = conj(chirp[d])
+// for 0 < d < N, zero between; evaluated in FP64 (the chirp phase from the
+// exact k^2 mod 2N, the FFT with directly evaluated twiddles) and rounded
+// once.  One allocation [chirp (N) | C (M)].
+int bluestein_tables(sarf_ctx *ctx, int N, int L, const float2 **chirp, const float2 **C) {
+    const int M = 1 << L;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->rc_blue.find(N);
+    if (it == ctx->rc_blue.end()) {
+        std::vector<double> cr(N), ci(N);
+        for (int k = 0; k < N; k++) {
+            const long q = ((long)k * k) % (2L * N);
+            const double a = kPi * (double)q / (double)N;
+            cr[k] = std::cos(a);
+            ci[k] = std::sin(a);
+        }
+        std::vector<double> hr(M, 0.0), hi(M, 0.0);
+        for (int d = 0; d < N; d++) {
+            hr[d] = cr[d];
+            hi[d] = -ci[d];
+            if (d > 0) {
+                hr[M - d] = cr[d];
+                hi[M - d] = -ci[d];
+            }
+        }
+        // forward FFT (e^{-j 2 pi t / M}), iterative radix 2
+        for (int i = 1, r = 0; i < M; i++) {
+            int bit = M >> 1;
+            for (; r & bit; bit >>= 1) r ^= bit;
+            r |= bit;
+            if (i < r) {
+                std::swap(hr[i], hr[r]);
+                std::swap(hi[i], hi[r]);
+            }
+        }
+        for (int len = 2; len <= M; len <<= 1) {
+            const int half = len / 2, step = M / len;
+            for (int st = 0; st < M; st += len)
+                for (int q = 0; q < half; q++) {
+                    const double a = -2.0 * kPi * (double)(q * step) / (double)M;
+                    const double wr = std::cos(a), wi = std::sin(a);
+                    const int i0 = st + q, i1 = i0 + half;
+                    const double tr = hr[i1] * wr - hi[i1] * wi, ti = hr[i1] * wi + hi[i1] * wr;
+                    hr[i1] = hr[i0] - tr;
+                    hi[i1] = hi[i0] - ti;
+                    hr[i0] += tr;
+                    hi[i0] += ti;
+                }
+        }
+        std::vector<float2> h((size_t)N + M);
+        for (int k = 0; k < N; k++) h[k] = make_float2((float)cr[k], (float)ci[k]);
+        for (int t = 0; t < M; t++) h[(size_t)N + t] = make_float2((float)hr[t], (float)hi[t]);
+        float2 *dv = nullptr;
+        hipError_t e = hipMalloc(&dv, sizeof(float2) * h.size());
+        if (e != hipSuccess)
+            return fail(SARF_ERR_NOMEM, "range_compress: Bluestein tables: %s", hipGetErrorString(e));
+        e = hipMemcpy(dv, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(dv);
+            return fail(SARF_ERR_HIP, "range_compress: Bluestein tables: %s", hipGetErrorString(e));
+        }
+        it = ctx->rc_blue.emplace(N, dv).first;
+    }
+    *chirp = it->second;
+    *C = it->second + N;
+    return SARF_OK;
+}
+
+template <int LOG2M>
+void launch_bluestein_t(const sarf_rc_desc *d, const float2 *ph, const float *taper,
+                        const double *f0, const double *r0, const float2 *chirp, const float2 *C,
+                        const float2 *tw, float2 *prof, hipStream_t s) {
+    constexpr int P = fft::rows_per_group(LOG2M);
+    hipLaunchKernelGGL(rc_bluestein<LOG2M>, dim3((d->Np + P - 1) / P),
+                       dim3(fft::group_threads(LOG2M)), 0, s, ph, d->K, d->Nfft, d->Np, d->flags,
+                       taper, f0, r0, chirp, C, tw, prof);
+}
+
+// Stockham passes of length N = 2^log2n through global memory (ping-pong
+// with `tmp` so the last pass lands in prof).
+template <typename T2>
+int launch_passes(const sarf_rc_desc *d, int log2n, const T2 *ph, const float *taper,
+                  const double *f0, const double *r0, const T2 *tw, T2 *prof, T2 *tmp,
+                  hipStream_t s) {
+    const int npass = (log2n & 1) + log2n / 2;
+    T2 *bufs[2] = {prof, tmp};
+    int cur = (npass % 2 == 1) ? 0 : 1;  // output buffer index of pass 0
+    const T2 *in = ph;
+    int Ns = 1, pass = 0;
+    if (log2n & 1) {
+        const long threads = (long)d->Np * (d->Nfft / 2);
+        hipLaunchKernelGGL((rc_pass<T2, 2>), dim3((threads + 255) / 256), dim3(256), 0, s, in,
+                           bufs[cur], d->Nfft, Ns, d->Np, tw, 1, d->K, d->flags, taper,
+                           npass == 1, f0, r0);
+        SARF_HIP(hipGetLastError());
+        in = bufs[cur];
+        cur ^= 1;
+        Ns *= 2;
+        pass++;
+    }
+    for (; pass < npass; pass++) {
+        const long threads = (long)d->Np * (d->Nfft / 4);
+        hipLaunchKernelGGL((rc_pass<T2, 4>), dim3((threads + 255) / 256), dim3(256), 0, s, in,
+                           bufs[cur], d->Nfft, Ns, d->Np, tw, pass == 0, d->K, d->flags, taper,
+                           pass == npass - 1, f0, r0);
+        SARF_HIP(hipGetLastError());
+        in = bufs[cur];
+        cur ^= 1;
+        Ns *= 4;
+    }
+    return SARF_OK;
+}
+
 int rc_pulses_per_group(int log2n) {
     const int T = (1 << log2n) / 16;
     return T >= 256 ? 1 : 256 / T;
@@ -458,13 +661,41 @@ int launch_range_compress(sarf_ctx *ctx, const sarf_rc_desc *d, const void *ph,
         return fail(SARF_ERR_INVALID, "range_compress: SARF_RC_TAPER without taper");
     if ((d->flags & SARF_RC_RAMP) && (!f0 || !r0))
         return fail(SARF_ERR_INVALID, "range_compress: SARF_RC_RAMP without f0/r0");
+    const bool f64 = (d->flags & SARF_RC_F64) != 0;
+    if (log2n < 0 && !f64 && d->Nfft <= kBluesteinMaxN) {
+        // not a power of two: Bluestein through two register-resident
+        // transforms of length M = 2^ceil(log2(2N - 1))
+        int L = 5;
+        while ((1 << L) < 2 * d->Nfft - 1) L++;
+        const float2 *chirp = nullptr, *C = nullptr, *tw = nullptr;
+        SARF_TRY(bluestein_tables(ctx, d->Nfft, L, &chirp, &C));
+        SARF_TRY(twiddles_f32(ctx, L, &tw));
+        const float2 *x = (const float2 *)ph;
+        float2 *y = (float2 *)prof;
+        switch (L) {
+            case 5: launch_bluestein_t<5>(d, x, taper, f0, r0, chirp, C, tw, y, s); break;
+            case 6: launch_bluestein_t<6>(d, x, taper, f0, r0, chirp, C, tw, y, s); break;
+            case 7: launch_bluestein_t<7>(d, x, taper, f0, r0, chirp, C, tw, y, s); break;
+            case 8: launch_bluestein_t<8>(d, x, taper, f0, r0, chirp, C, tw, y, s); break;
+            case 9: launch_bluestein_t<9>(d, x, taper, f0, r0, chirp, C, tw, y, s); break;
+            case 10: launch_bluestein_t<10>(d, x, taper, f0, r0, chirp, C, tw, y, s); break;
+            case 11: launch_bluestein_t<11>(d, x, taper, f0, r0, chirp, C, tw, y, s); break;
+            case 12: launch_bluestein_t<12>(d, x, taper, f0, r0, chirp, C, tw, y, s); break;
+            case 13: launch_bluestein_t<13>(d, x, taper, f0, r0, chirp, C, tw, y, s); break;
+            case 14: launch_bluestein_t<14>(d, x, taper, f0, r0, chirp, C, tw, y, s); break;
+            default: return fail(SARF_ERR_UNSUPPORTED, "range_compress: Nfft=%d", d->Nfft);
+        }
+        SARF_HIP(hipGetLastError());
+        return SARF_OK;
+    }
     if (log2n < 0) {
-        // not a power of two: direct DFT with an exact twiddle index
+        // not a power of two, beyond the Bluestein range (or FP64): direct
+        // DFT with an exact twiddle index
         if (d->Nfft > (1 << 20) || d->Np > 65535)
             return fail(SARF_ERR_UNSUPPORTED, "range_compress: Nfft=%d / Np=%d too large for the "
                         "direct transform", d->Nfft, d->Np);
         dim3 grid((d->Nfft + 255) / 256, d->Np);
-        if (d->flags & SARF_RC_F64) {
+        if (f64) {
             const double2 *tw = nullptr;
             SARF_TRY(twiddles_any_f64(ctx, d->Nfft, &tw));
             hipLaunchKernelGGL((rc_direct<double2, double>), grid, dim3(256), 0, s,
@@ -480,46 +711,23 @@ int launch_range_compress(sarf_ctx *ctx, const sarf_rc_desc *d, const void *ph,
         SARF_HIP(hipGetLastError());
         return SARF_OK;
     }
-    if (d->flags & SARF_RC_F64) {
+    if (f64 || log2n > 14 || log2n < 5) {
         if (log2n < 1 || log2n > 24)
-            return fail(SARF_ERR_UNSUPPORTED, "range_compress f64: Nfft=%d", d->Nfft);
-        const double2 *tw = nullptr;
-        SARF_TRY(twiddles_f64(ctx, log2n, &tw));
-        const size_t bytes = sizeof(double2) * (size_t)d->Nfft * d->Np;
+            return fail(SARF_ERR_UNSUPPORTED, "range_compress: Nfft=%d", d->Nfft);
         void *tmp = nullptr;
-        SARF_TRY(scratch_get(ctx, "rc64.tmp", bytes, &tmp));
-        // passes: radix-2 first when log2n is odd, then radix-4
-        int npass = (log2n & 1) + log2n / 2;
-        // ping-pong so that the last pass lands in prof
-        double2 *bufs[2] = {(double2 *)prof, (double2 *)tmp};
-        int cur = (npass % 2 == 1) ? 0 : 1;  // output buffer index of pass 0
-        const double2 *in = (const double2 *)ph;
-        int Ns = 1, pass = 0;
-        if (log2n & 1) {
-            const long threads = (long)d->Np * (d->Nfft / 2);
-            hipLaunchKernelGGL(rc_pass_f64<2>, dim3((threads + 255) / 256), dim3(256), 0, s, in,
-                               bufs[cur], d->Nfft, Ns, d->Np, tw, 1, d->K, d->flags, taper,
-                               npass == 1, f0, r0);
-            SARF_HIP(hipGetLastError());
-            in = bufs[cur];
-            cur ^= 1;
-            Ns *= 2;
-            pass++;
+        if (f64) {
+            const double2 *tw = nullptr;
+            SARF_TRY(twiddles_f64(ctx, log2n, &tw));
+            SARF_TRY(scratch_get(ctx, "rc64.tmp", sizeof(double2) * (size_t)d->Nfft * d->Np, &tmp));
+            return launch_passes<double2>(d, log2n, (const double2 *)ph, taper, f0, r0, tw,
+                                          (double2 *)prof, (double2 *)tmp, s);
         }
-        for (; pass < npass; pass++) {
-            const long threads = (long)d->Np * (d->Nfft / 4);
-            hipLaunchKernelGGL(rc_pass_f64<4>, dim3((threads + 255) / 256), dim3(256), 0, s, in,
-                               bufs[cur], d->Nfft, Ns, d->Np, tw, pass == 0, d->K, d->flags,
-                               taper, pass == npass - 1, f0, r0);
-            SARF_HIP(hipGetLastError());
-            in = bufs[cur];
-            cur ^= 1;
-            Ns *= 4;
-        }
-        return SARF_OK;
+        const float2 *tw = nullptr;
+        SARF_TRY(twiddles_f32(ctx, log2n, &tw));
+        SARF_TRY(scratch_get(ctx, "rc32.tmp", sizeof(float2) * (size_t)d->Nfft * d->Np, &tmp));
+        return launch_passes<float2>(d, log2n, (const float2 *)ph, taper, f0, r0, tw,
+                                     (float2 *)prof, (float2 *)tmp, s);
     }
-    if (log2n < 5 || log2n > 14)
-        return fail(SARF_ERR_UNSUPPORTED, "range_compress: Nfft=%d outside [32,16384]", d->Nfft);
     const float2 *tw = nullptr;
     SARF_TRY(twiddles_f32(ctx, log2n, &tw));
     const float2 *x = (const float2 *)ph;
@@ -551,6 +759,7 @@ extern "C" int sarf_range_compress_scenes(sarf_ctx *ctx, int nscenes,
                                           const sarf_rc_scene *scenes, void *stream) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    StreamOrder order_(ctx, (hipStream_t)stream);
     if (nscenes < 1 || !scenes)
         return fail(SARF_ERR_INVALID, "sarf_range_compress_scenes: no scenes");
     // class key = log2n * 2 + (16 sample rows ? 1 : 0) -> scene indices
@@ -601,33 +810,20 @@ extern "C" int sarf_range_compress_scenes(sarf_ctx *ctx, int nscenes,
     for (const Launch &l : launches) SARF_TRY(twiddles_f32(ctx, l.log2n, &tws[l.log2n]));
 
     const size_t bytes = sizeof(RcJob) * jobs.size();
-    void *buf = nullptr;
-    SARF_TRY(scratch_get(ctx, "rcs.jobs", bytes, &buf));
-    RcJob *djobs = (RcJob *)buf;
     hipStream_t s = (hipStream_t)stream;
-    // The job list depends only on the caller's descriptors: when they match
-    // the last upload into this buffer, launch without copying or waiting,
-    // so the call can be captured into a graph once it has run outside.
+    // The job list depends only on the caller's descriptors: a list seen
+    // before has its own device copy, so the call launches without copying
+    // or waiting (and can be captured into a graph once it has run outside).
     std::vector<unsigned char> key(sizeof(sarf_rc_scene) * (size_t)nscenes);
     std::memcpy(key.data(), scenes, key.size());
-    bool cached;
-    {
-        std::lock_guard<std::mutex> lk(ctx->mu);
-        cached = ctx->rcs_buf == buf && ctx->rcs_key == key;
-    }
-    if (!cached) {
-        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-        SARF_HIP(hipStreamIsCapturing(s, &cap));
-        if (cap != hipStreamCaptureStatusNone)
-            return fail(SARF_ERR_INVALID,
-                        "sarf_range_compress_scenes: a new scene list cannot be uploaded during "
-                        "stream capture; make the same call once before capturing");
+    void *buf = nullptr;
+    bool fresh = false;
+    SARF_TRY(list_buffer(ctx, 'r', key, bytes, s, "sarf_range_compress_scenes", &buf, &fresh));
+    RcJob *djobs = (RcJob *)buf;
+    if (fresh) {
         // pageable source: wait for the copy before the host vector goes away
         SARF_HIP(hipMemcpyAsync(djobs, jobs.data(), bytes, hipMemcpyHostToDevice, s));
         SARF_HIP(hipStreamSynchronize(s));
-        std::lock_guard<std::mutex> lk(ctx->mu);
-        ctx->rcs_buf = buf;
-        ctx->rcs_key = std::move(key);
     }
     for (const Launch &l : launches) {
         const RcJob *jl = djobs + l.first;

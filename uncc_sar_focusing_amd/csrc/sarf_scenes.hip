@@ -119,6 +119,7 @@ extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scen
                               void *stream) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    StreamOrder order_(ctx, (hipStream_t)stream);
     if (nscenes < 1 || !scenes) return fail(SARF_ERR_INVALID, "sarf_bp_scenes: no scenes");
     std::vector<SceneDev> host(nscenes);
     struct Item {
@@ -152,39 +153,27 @@ extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scen
     std::vector<int2> hitems(items.size());
     for (size_t i = 0; i < items.size(); i++) hitems[i] = make_int2(items[i].scene, items[i].tile);
     const size_t bytes_sc = sizeof(SceneDev) * nscenes, bytes_it = sizeof(int2) * hitems.size();
-    void *buf = nullptr;
-    SARF_TRY(scratch_get(ctx, "scenes.desc", bytes_sc + bytes_it + 256, &buf));
-    SceneDev *dsc = (SceneDev *)buf;
-    int2 *ditems = (int2 *)(((uintptr_t)((char *)buf + bytes_sc) + 15) & ~(uintptr_t)15);
-    unsigned int *counter = (unsigned int *)(((uintptr_t)((char *)ditems + bytes_it) + 15) & ~(uintptr_t)15);
     hipStream_t s = (hipStream_t)stream;
-    // The work list depends only on the caller's descriptors and the tile:
-    // when both match the last upload into this buffer, launch again
-    // without copying (and without waiting, so the call can be captured
-    // into a graph once the list has been uploaded outside the capture).
+    // The work list depends only on the caller's descriptors and the tile: a
+    // list seen before has its own device copy, so the call launches without
+    // copying or waiting (and can be captured into a graph once it has run
+    // outside the capture).
     std::vector<unsigned char> key(sizeof(int) + sizeof(sarf_scene) * (size_t)nscenes);
     std::memcpy(key.data(), &tid, sizeof(int));
     std::memcpy(key.data() + sizeof(int), scenes, sizeof(sarf_scene) * (size_t)nscenes);
-    bool cached;
-    {
-        std::lock_guard<std::mutex> lk(ctx->mu);
-        cached = ctx->scenes_buf == buf && ctx->scenes_key == key;
-    }
-    if (!cached) {
-        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-        SARF_HIP(hipStreamIsCapturing(s, &cap));
-        if (cap != hipStreamCaptureStatusNone)
-            return fail(SARF_ERR_INVALID,
-                        "sarf_bp_scenes: a new scene list cannot be uploaded during stream "
-                        "capture; make the same call once before capturing");
+    void *buf = nullptr;
+    bool fresh = false;
+    SARF_TRY(list_buffer(ctx, 's', key, bytes_sc + bytes_it + 256, s, "sarf_bp_scenes", &buf,
+                         &fresh));
+    SceneDev *dsc = (SceneDev *)buf;
+    int2 *ditems = (int2 *)(((uintptr_t)((char *)buf + bytes_sc) + 15) & ~(uintptr_t)15);
+    unsigned int *counter = (unsigned int *)(((uintptr_t)((char *)ditems + bytes_it) + 15) & ~(uintptr_t)15);
+    if (fresh) {
         // pageable source: the copy completes before returning, so the host
         // vectors may go out of scope
         SARF_HIP(hipMemcpyAsync(dsc, host.data(), bytes_sc, hipMemcpyHostToDevice, s));
         SARF_HIP(hipMemcpyAsync(ditems, hitems.data(), bytes_it, hipMemcpyHostToDevice, s));
         SARF_HIP(hipStreamSynchronize(s));
-        std::lock_guard<std::mutex> lk(ctx->mu);
-        ctx->scenes_buf = buf;
-        ctx->scenes_key = std::move(key);
     }
     const int cus = ctx->cu_count > 0 ? ctx->cu_count : 256;
     const int nit = (int)hitems.size();
@@ -212,6 +201,7 @@ extern "C" int sarf_autotune_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene
                                     int *best_tile, float *best_ms, void *stream) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    StreamOrder order_(ctx, (hipStream_t)stream);
     if (nscenes < 1 || !scenes || !best_tile || reps < 1 || reps > 100)
         return fail(SARF_ERR_INVALID, "sarf_autotune_scenes: bad arguments");
     hipStream_t s = (hipStream_t)stream;

@@ -181,6 +181,7 @@ extern "C" int sarf_synth_phase_history_nufft(sarf_ctx *ctx, int K, int Np, int 
                                               void *stream) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    StreamOrder order_(ctx, (hipStream_t)stream);
     if (K < 2 || Np < 1 || S < 0)
         return fail(SARF_ERR_INVALID, "synth_nufft: bad sizes K=%d Np=%d S=%d", K, Np, S);
     if (S == 0) return SARF_OK;
@@ -228,6 +229,7 @@ extern "C" int sarf_synth_phase_history(sarf_ctx *ctx, int K, int Np, int S, con
                                         const double *f0, double df, void *ph, void *stream) {
     clear_error();
     SARF_TRY(use_device(ctx));
+    StreamOrder order_(ctx, (hipStream_t)stream);
     if (K < 1 || Np < 1 || S < 0 || Np > 65535)
         return fail(SARF_ERR_INVALID, "synth: bad sizes K=%d Np=%d S=%d", K, Np, S);
     if (S == 0) return SARF_OK;
