@@ -174,6 +174,20 @@ int sarf_backproject(sarf_ctx *ctx, const sarf_bp_desc *desc,
                      void *img,          /* device, complex Ny x Nx (float2; double2 for F64) */
                      void *stream);
 
+/* Allocation-free sarf_backproject: the same results, but no device memory */
+/* is allocated or freed in the call (no hipMalloc/hipFree, no context      */
+/* scratch).  Partial images of pulse splits go to a 64 MiB device array    */
+/* that the library holds per GPU for the life of the process: an automatic */
+/* or tuned split count is lowered until Nx*Ny*8*splits fits, and an        */
+/* explicit desc->pulse_split that does not fit returns SARF_ERR_INVALID.   */
+/* Every call on one GPU shares that array: calls that overlap on a GPU     */
+/* (other streams, other threads, other contexts) race.  No counterpart in  */
+/* the reference; it is for callers that cannot allocate -- code under HIP  */
+/* graph capture, tight serving loops, harnesses that forbid allocation.    */
+int sarf_backproject_noalloc(sarf_ctx *ctx, const sarf_bp_desc *desc,
+                             const void *prof, const double *ant, const double *r0,
+                             const double *f0, void *img, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Matched-filter image formation (exact, O(K) per pixel and pulse):        */
 /*   img[p] += sum_n sum_k ph[n*K + k] exp(j 4 pi f_{n,k} dR_n(p) / c),     */

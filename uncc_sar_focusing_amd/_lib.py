@@ -92,6 +92,7 @@ SIGNATURES = {
     "sarf_pga_scenes": (_i, [_vp, _i, C.c_void_p, _i, _d, _d, _vp]),
     "sarf_grid_search": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                               _vp, _vp, _vp, _vp]),
+    "sarf_backproject_noalloc": (_i, [_vp, C.POINTER(BpDesc), _vp, _vp, _vp, _vp, _vp, _vp]),
     "sarf_grid_covariance": (_i, [_vp, _vp, C.c_long, _vp]),
     "sarf_fit_linear": (_i, [_vp, _i, _i, _vp]),
     "sarf_fit_quadratic": (_i, [_vp, _i, _i, _vp]),

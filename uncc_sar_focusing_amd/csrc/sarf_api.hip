@@ -217,6 +217,7 @@ int sarf_ctx_create(int device, sarf_ctx **out) {
         return fail(SARF_ERR_NODEVICE, "device %d is %s; libsarf is built for gfx950 only",
                     device, prop.gcnArchName);
     SARF_HIP(hipSetDevice(device));
+    SARF_TRY(bp_noalloc_init(device));
     hipEvent_t ev = nullptr;
     SARF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     sarf_ctx *ctx = new sarf_ctx();

@@ -546,3 +546,59 @@ extern "C" int sarf_tune_table_set(sarf_ctx *ctx, const sarf_tune_entry *entries
     return SARF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free backprojection.  Its partial images live in `scratch`, a
+// file-scope device array: every GPU the library runs on holds one copy of
+// kNoAllocScratchBytes until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it only names the symbol); it uses
+// the device address hipGetSymbolAddress returned for each device, which
+// sarf_ctx_create records through bp_noalloc_init.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+float2 *g_noalloc_ptr[kNoAllocMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) float2 scratch[kNoAllocScratchBytes / sizeof(float2)];
+
+namespace sarf {
+
+// Records this device's address of `scratch`; the device must be current.
+int bp_noalloc_init(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices)
+        return fail(SARF_ERR_NODEVICE, "device %d: at most %d devices", device,
+                    kNoAllocMaxDevices);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return SARF_OK;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(SARF_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                    hipGetErrorString(e));
+    g_noalloc_ptr[device] = (float2 *)p;
+    return SARF_OK;
+}
+
+}  // namespace sarf
+
+extern "C" int sarf_backproject_noalloc(sarf_ctx *ctx, const sarf_bp_desc *desc,
+                                        const void *prof, const double *ant, const double *r0,
+                                        const double *f0, void *img, void *stream) {
+    clear_error();
+    SARF_TRY(use_device(ctx));
+    StreamOrder order_(ctx, (hipStream_t)stream);
+    if (!desc || !prof || !ant || !r0 || !f0 || !img)
+        return fail(SARF_ERR_INVALID, "sarf_backproject_noalloc: null argument");
+    float2 *part = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        if (ctx->device >= 0 && ctx->device < kNoAllocMaxDevices) part = g_noalloc_ptr[ctx->device];
+    }
+    if (!part)
+        return fail(SARF_ERR_INVALID, "sarf_backproject_noalloc: no scratch address for device %d",
+                    ctx->device);
+    return launch_backproject_ext(ctx, desc, prof, ant, r0, f0, img, (hipStream_t)stream, part,
+                                  kNoAllocScratchBytes);
+}

@@ -130,6 +130,8 @@ int launch_backproject(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof,
 int launch_backproject_ext(sarf_ctx *ctx, const sarf_bp_desc *d, const void *prof,
                            const double *ant, const double *r0, const double *f0, void *img,
                            hipStream_t stream, float2 *ext_part, size_t ext_bytes);
+// Records the current device's address of the allocation-free entry's scratch.
+int bp_noalloc_init(int device);
 // img[i] (+)= sum over s in order of part[s*stride + i], i < n (deterministic).
 int launch_reduce_partials(const float2 *part, long stride, int splits, long n, int accumulate,
                            float2 *img, hipStream_t stream);
