@@ -249,6 +249,25 @@ def c0_leg(ctx, torch, cpu: bool):
     def rel(a):
         return float(np.linalg.norm(a - ref) / np.linalg.norm(ref))
 
+    # range compression at Nfft = K (the reference callback's default length,
+    # not a power of two: Bluestein) beside the power-of-two Nfft
+    import oracle
+
+    prof_k = ctx.range_compress(phd, geom.K)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(n):
+        ctx.range_compress(phd, geom.K, out=prof_k)
+    e1.record()
+    torch.cuda.synchronize()
+    rc_k_ms = e0.elapsed_time(e1) / n
+    e0.record()
+    for _ in range(n):
+        ctx.range_compress(phd, cfg.nfft, out=prof)
+    e1.record()
+    torch.cuda.synchronize()
+    rc_ms = e0.elapsed_time(e1) / n
+    rc_k_err = oracle.rel_l2(prof_k.cpu().numpy(), oracle.range_compress_f64(ph, geom.K))
     # exact matched filter (O(K) per pixel*pulse) on the same data
     mimg = ctx.matched_filter(phd, ant, r0, f0, geom.df, grid)
     torch.cuda.synchronize()
@@ -265,6 +284,9 @@ def c0_leg(ctx, torch, cpu: bool):
         "gpu_ms_rc_bp": round(ms, 4),
         "gpu_Gpixel_pulses_per_s": round(pp / (ms * 1e-3) / 1e9, 2),
         "gpu_rel_l2_vs_fp64": rel(gimg),
+        "rc_ms_nfft_eq_K": round(rc_k_ms, 4),
+        "rc_ms_nfft_pow2": round(rc_ms, 4),
+        "rc_nfft_eq_K_rel_l2_vs_oracle": rc_k_err,
         "gpu_peaks_match": local_peaks_np(np.abs(gimg), grid, tg.pos)
         == local_peaks_np(np.abs(ref), grid, tg.pos),
         "matched_filter_gpu_ms": round(mf_ms, 3),

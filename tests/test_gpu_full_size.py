@@ -15,7 +15,15 @@ size-independent properties of backprojection.
   any of them; the best candidate's image is within 1e-4 rel-L2 of FP64.
 - C4 (64 uneven scenes): batched range compression equals per-scene calls bit
   for bit, the one-launch BP of all scenes is within 1e-4 rel-L2 of each
-  scene's FP64 BP, and batched PGA equals per-image PGA bit for bit.
+  scene's FP64 BP with the same brightest pixel, and batched PGA equals
+  per-image PGA bit for bit.
+
+Each also closes the chain to the CPU oracle at the full image size: the
+GPU image of the first 128 pulses (range compression + mixed backprojection)
+against the oracle's FP64 range compression + backprojection of the same
+phase history -- backprojection is linear in the pulses, so a pulse subset
+checks the full-size image geometry at CPU cost -- within 1e-4 rel-L2 with
+the same brightest pixel (C1, the C2 focused scene, the largest C4 scene).
 """
 import numpy as np
 import pytest
@@ -31,6 +39,18 @@ DEV = "cuda:0"
 
 def _t(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def _oracle_subset(ctx, ph, prof, ant, r0, f0, grid, maxWr, nfft, los, n=128):
+    import oracle
+
+    sl = slice(0, n)
+    img = ctx.backproject(prof[sl].contiguous(), ant[sl].contiguous(), r0[sl].contiguous(),
+                          f0[sl].contiguous(), grid, maxWr, los=los).cpu().numpy()
+    ref = oracle.bp_f64(oracle.range_compress_f64(ph[sl].cpu().numpy(), nfft),
+                        ant[sl].cpu().numpy(), r0[sl].cpu().numpy(), f0[sl].cpu().numpy(), grid,
+                        maxWr)
+    return oracle.rel_l2(img, ref), peak_index(img) == peak_index(ref)
 
 
 def _rel(a, b):
@@ -79,6 +99,9 @@ def test_c1_full_size_parity(ctx):
     e_hom = _rel(scaled, img * c)
     print(f"C1 additivity {e_add:.3e}, homogeneity {e_hom:.3e}")
     assert e_add <= 1e-6 and e_hom <= 1e-6
+    e_cpu, same_peak = _oracle_subset(ctx, ph, prof, ant, r0, f0, grid, geom.maxWr, cfg.nfft, los)
+    print(f"C1 first 128 pulses vs CPU FP64 oracle: rel-L2 {e_cpu:.3e}, same peak {same_peak}")
+    assert e_cpu <= 1e-4 and same_peak
 
 
 def test_c2_full_size_grid_search_matches_single_images(ctx):
@@ -111,6 +134,14 @@ def test_c2_full_size_grid_search_matches_single_images(ctx):
     e_bp = _rel(img, img64)
     print(f"C2 best candidate BP rel-L2 mixed vs FP64: {e_bp:.3e}")
     assert e_bp <= 1e-4
+    assert peak_index(img.cpu().numpy()) == peak_index(img64.cpu().numpy())
+    del prof64, img64
+    # the focused (true-trajectory) scene against the CPU oracle
+    ant_t = _t(sc.true_geom.ant)
+    e_cpu, same_peak = _oracle_subset(ctx, sc.ph, prof, ant_t, sc.r0, sc.f0, grid, g.maxWr,
+                                      sc.cfg.nfft, los)
+    print(f"C2 first 128 pulses vs CPU FP64 oracle: rel-L2 {e_cpu:.3e}, same peak {same_peak}")
+    assert e_cpu <= 1e-4 and same_peak
 
 
 def test_c4_full_size_batch_matches_per_scene_paths(ctx):
@@ -130,9 +161,17 @@ def test_c4_full_size_batch_matches_per_scene_paths(ctx):
         p64 = ctx.range_compress(s.ph.to(torch.complex128), s.nfft)
         i64 = ctx.backproject(p64, s.ant, s.r0, s.f0, s.grid, s.geom.maxWr, precision="f64")
         worst = max(worst, _rel(imgs[i], i64))
+        assert int(torch.argmax(imgs[i].abs())) == int(torch.argmax(i64.abs())), i
         del p64, i64
     print(f"C4 batched BP worst rel-L2 vs FP64 over 64 scenes: {worst:.3e}")
     assert worst <= 1e-4
+    big = max(range(len(scenes)), key=lambda i: scenes[i].pixel_pulses)
+    sb = scenes[big]
+    e_cpu, same_peak = _oracle_subset(ctx, sb.ph, profs[big], sb.ant, sb.r0, sb.f0, sb.grid,
+                                      sb.geom.maxWr, sb.nfft, sb.geom.los(sb.grid.x0, sb.grid.y0))
+    print(f"C4 largest scene ({big}) first 128 pulses vs CPU FP64 oracle: rel-L2 {e_cpu:.3e}, "
+          f"same peak {same_peak}")
+    assert e_cpu <= 1e-4 and same_peak
     singles = [im.clone() for im in imgs]
     batched = ctx.pga_scenes(imgs, 5)
     for i in range(len(scenes)):

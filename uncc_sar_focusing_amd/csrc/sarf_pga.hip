@@ -219,6 +219,8 @@ __device__ void argmax_body(const PgaImage &p, int y) {
 // Two-pass column sums over rows.  Pass 1, block (bx, chunk):
 // pa[chunk*N + x] = sum over the chunk's rows of |At[y][x]|^2 (mode 0), or
 // for G (mode 1) num/den parts of phi_dot: Im(conj G_k G_{k+1}), |G_k|^2.
+// Pass 2 (the sum over chunks, in chunk order) opens the window and
+// integrate steps (window_fused_body, integrate_fused_body).
 __device__ void col_partial_body(const PgaImage &p, int mode, int bx, int chunk) {
     if (stopped(p)) return;
     __shared__ double ra[4][64], rb[4][64];
@@ -248,22 +250,6 @@ __device__ void col_partial_body(const PgaImage &p, int mode, int bx, int chunk)
         p.pa[(size_t)chunk * N + x] = (ra[0][lx] + ra[1][lx]) + (ra[2][lx] + ra[3][lx]);
         if (mode == 1) p.pb[(size_t)chunk * N + x] = (rb[0][lx] + rb[1][lx]) + (rb[2][lx] + rb[3][lx]);
     }
-}
-
-// Pass 2, block bx: w[x] = sum of the chunk parts (mode 0: energy), or
-// phidot[k] = num / den for k < N-1 (mode 1).
-__device__ void col_final_body(const PgaImage &p, int mode, int bx) {
-    if (stopped(p)) return;
-    const int N = p.Nx;
-    const int x = bx * kThreads + threadIdx.x;
-    if (x >= N) return;
-    double a = 0.0, b = 0.0;
-    for (int c = 0; c < kRowChunks; c++) {
-        a += p.pa[(size_t)c * N + x];
-        if (mode == 1) b += p.pb[(size_t)c * N + x];
-    }
-    if (mode == 0) p.w[x] = a;
-    else if (x < N - 1) p.phidot[x] = b > 0.0 ? a / b : 0.0;
 }
 
 // One workgroup of kIntThreads: window limits from the column energy
@@ -337,8 +323,8 @@ __device__ inline double block_sum_1024(double v, double *red) {
 // rms -> rms[it], af_ph += phi; writes e^{-j phi} (detrended) for the
 // correction.  unwrap of a cumulative sum is the cumulative sum of the
 // unwrapped steps, so phi is one block-wide prefix scan of unwrap_step.
-__device__ void integrate_body(const PgaImage &p, int it) {
-    if (stopped(p)) return;
+__device__ bool integrate_body(const PgaImage &p, int it) {
+    if (stopped(p)) return false;
     __shared__ double sc[kIntThreads];
     __shared__ double red[kIntThreads];
     const int Nx = p.Nx, tid = threadIdx.x;
@@ -392,7 +378,7 @@ __device__ void integrate_body(const PgaImage &p, int it) {
     // before it is applied
     if (p.rms_stop > 0.0 && rms < p.rms_stop) {
         if (tid == 0) *p.stop = 1;
-        return;
+        return false;
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -405,6 +391,7 @@ __device__ void integrate_body(const PgaImage &p, int it) {
             p.ephi[i] = make_double2(cs, -sn);
         }
     }
+    return true;
 }
 
 // Short rows (M <= 512): one workgroup of kThreads per row, the row in LDS
@@ -623,6 +610,53 @@ __device__ void apply_spectrum_body(const PgaImage &p) {
     for (int m = threadIdx.x; m < M; m += blockDim.x) p.B[m] = buf[m];
 }
 
+// Fused steps (fewer launches per iteration, same arithmetic):
+//   stats:     block b < Ny: argmax of row b; else a block of the column-energy
+//              partial sums (col_partial_body, mode 0);
+//   window:    the column energy from its partials, then the window
+//              (one workgroup of kIntThreads);
+//   integrate: phi_dot from the phase-difference partials, the integration
+//              and detrend, and -- if the estimate is applied -- the correction
+//              kernel's spectrum (one workgroup of kIntThreads, dynamic LDS of
+//              M float2).
+__device__ void stats_body(const PgaImage &p, int b) {
+    if (b < p.Ny) {
+        argmax_body(p, b);
+    } else {
+        const int l = b - p.Ny, gx = (p.Nx + 63) / 64;
+        col_partial_body(p, 0, l % gx, l / gx);
+    }
+}
+
+__device__ void window_fused_body(const PgaImage &p, double db_down) {
+    if (stopped(p)) return;
+    const int N = p.Nx;
+    for (int x = threadIdx.x; x < N; x += blockDim.x) {
+        double a = 0.0;
+        for (int c = 0; c < kRowChunks; c++) a += p.pa[(size_t)c * N + x];
+        p.w[x] = a;
+    }
+    __syncthreads();
+    window_body(p, db_down);
+}
+
+__device__ void integrate_fused_body(const PgaImage &p, int it) {
+    if (stopped(p)) return;
+    const int N = p.Nx;
+    for (int x = threadIdx.x; x < N - 1; x += blockDim.x) {
+        double a = 0.0, b = 0.0;
+        for (int c = 0; c < kRowChunks; c++) {
+            a += p.pa[(size_t)c * N + x];
+            b += p.pb[(size_t)c * N + x];
+        }
+        p.phidot[x] = b > 0.0 ? a / b : 0.0;
+    }
+    __syncthreads();
+    if (!integrate_body(p, it)) return;
+    __syncthreads();
+    apply_spectrum_body(p);
+}
+
 // Blocks of a row step for an image of Ny rows: one workgroup per row up to
 // L = 9, rows_per_group rows per workgroup from L = 10.
 int row_blocks(int L, int Ny) {
@@ -640,21 +674,15 @@ __global__ __launch_bounds__(kThreads) void k1_transpose(PgaImage p, int forward
     if (forward) transpose_body(p.img, p.Nx, p.Ny, p.At, blockIdx.x, blockIdx.y);
     else transpose_body(p.At, p.Ny, p.Nx, p.img, blockIdx.x, blockIdx.y);
 }
-__global__ __launch_bounds__(kThreads) void k1_argmax(PgaImage p) { argmax_body(p, blockIdx.x); }
+__global__ __launch_bounds__(kThreads) void k1_stats(PgaImage p) { stats_body(p, blockIdx.x); }
 __global__ __launch_bounds__(kThreads) void k1_col_partial(PgaImage p, int mode) {
     col_partial_body(p, mode, blockIdx.x, blockIdx.y);
 }
-__global__ __launch_bounds__(kThreads) void k1_col_final(PgaImage p, int mode) {
-    col_final_body(p, mode, blockIdx.x);
-}
 __global__ __launch_bounds__(kIntThreads) void k1_window(PgaImage p, double db_down) {
-    window_body(p, db_down);
+    window_fused_body(p, db_down);
 }
 __global__ __launch_bounds__(kIntThreads) void k1_integrate(PgaImage p, int it) {
-    integrate_body(p, it);
-}
-__global__ __launch_bounds__(kThreads) void k1_apply_spectrum(PgaImage p) {
-    apply_spectrum_body(p);
+    integrate_fused_body(p, it);
 }
 __global__ __launch_bounds__(kThreads) void k1_rows_lds(PgaImage p, int apply) {
     if (apply) row_apply_lds_body(p, blockIdx.x);
@@ -681,7 +709,7 @@ __device__ __forceinline__ int owner(const int *__restrict__ off, int n, int blk
 }
 
 // Per-image block offsets, one array of n + 1 per step kind.
-enum OffKind { kOffTranspose = 0, kOffRows, kOffColPartial, kOffColFinal, kOffKinds };
+enum OffKind { kOffTranspose = 0, kOffRows, kOffColPartial, kOffKinds };
 
 struct PgaBatchDev {
     const PgaImage *img;  // n images
@@ -703,10 +731,21 @@ __global__ __launch_bounds__(kThreads) void kb_transpose(PgaBatchDev b, int forw
     if (forward) transpose_body(p.img, R, C, p.At, l % gx, l / gx);
     else transpose_body(p.At, R, C, p.img, l % gx, l / gx);
 }
-__global__ __launch_bounds__(kThreads) void kb_argmax(PgaBatchDev b) {
-    const int *off = b.off + kOffRows * (b.n + 1);
-    const int i = owner(off, b.n, blockIdx.x);
-    argmax_body(b.img[i], blockIdx.x - off[i]);
+// Blocks [0, rows): argmax of every image's rows; then the column-energy
+// partial blocks of every image.
+__global__ __launch_bounds__(kThreads) void kb_stats(PgaBatchDev b, int rows) {
+    if ((int)blockIdx.x < rows) {
+        const int *off = b.off + kOffRows * (b.n + 1);
+        const int i = owner(off, b.n, blockIdx.x);
+        argmax_body(b.img[i], blockIdx.x - off[i]);
+    } else {
+        const int *off = b.off + kOffColPartial * (b.n + 1);
+        const int bb = blockIdx.x - rows;
+        const int i = owner(off, b.n, bb);
+        const PgaImage &p = b.img[i];
+        const int l = bb - off[i], gx = (p.Nx + 63) / 64;
+        col_partial_body(p, 0, l % gx, l / gx);
+    }
 }
 __global__ __launch_bounds__(kThreads) void kb_col_partial(PgaBatchDev b, int mode) {
     const int *off = b.off + kOffColPartial * (b.n + 1);
@@ -715,19 +754,11 @@ __global__ __launch_bounds__(kThreads) void kb_col_partial(PgaBatchDev b, int mo
     const int l = blockIdx.x - off[i], gx = (p.Nx + 63) / 64;
     col_partial_body(p, mode, l % gx, l / gx);
 }
-__global__ __launch_bounds__(kThreads) void kb_col_final(PgaBatchDev b, int mode) {
-    const int *off = b.off + kOffColFinal * (b.n + 1);
-    const int i = owner(off, b.n, blockIdx.x);
-    col_final_body(b.img[i], mode, blockIdx.x - off[i]);
-}
 __global__ __launch_bounds__(kIntThreads) void kb_window(PgaBatchDev b, double db_down) {
-    window_body(b.img[blockIdx.x], db_down);
+    window_fused_body(b.img[blockIdx.x], db_down);
 }
 __global__ __launch_bounds__(kIntThreads) void kb_integrate(PgaBatchDev b, int it) {
-    integrate_body(b.img[blockIdx.x], it);
-}
-__global__ __launch_bounds__(kThreads) void kb_apply_spectrum(PgaBatchDev b) {
-    apply_spectrum_body(b.img[blockIdx.x]);
+    integrate_fused_body(b.img[blockIdx.x], it);
 }
 // Row step of one transform length: grp[g0 .. g0 + m) are the images' ids,
 // grp[g0 + m .. g0 + 2m] their block offsets.
@@ -889,17 +920,13 @@ extern "C" int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *d, void *img, double
         // img is Nx rows (x) of Ny values; At is Ny rows of Nx values
         hipLaunchKernelGGL(k1_transpose, dim3((p.Ny + 63) / 64, (p.Nx + 63) / 64), dim3(kThreads),
                            0, s, p, 1);
-        const dim3 gcp((p.Nx + 63) / 64, kRowChunks), gcf((p.Nx + kThreads - 1) / kThreads);
+        const dim3 gcp((p.Nx + 63) / 64, kRowChunks);
         for (int it = 0; it < d->iters; it++) {
-            hipLaunchKernelGGL(k1_argmax, dim3(p.Ny), dim3(kThreads), 0, s, p);
-            hipLaunchKernelGGL(k1_col_partial, gcp, dim3(kThreads), 0, s, p, 0);
-            hipLaunchKernelGGL(k1_col_final, gcf, dim3(kThreads), 0, s, p, 0);
+            hipLaunchKernelGGL(k1_stats, dim3(p.Ny + gcp.x * gcp.y), dim3(kThreads), 0, s, p);
             hipLaunchKernelGGL(k1_window, dim3(1), dim3(kIntThreads), 0, s, p, d->db_down);
             SARF_TRY(k1_row_step(p, 0, s));
             hipLaunchKernelGGL(k1_col_partial, gcp, dim3(kThreads), 0, s, p, 1);
-            hipLaunchKernelGGL(k1_col_final, gcf, dim3(kThreads), 0, s, p, 1);
-            hipLaunchKernelGGL(k1_integrate, dim3(1), dim3(kIntThreads), 0, s, p, it);
-            hipLaunchKernelGGL(k1_apply_spectrum, dim3(1), dim3(kThreads), lds, s, p);
+            hipLaunchKernelGGL(k1_integrate, dim3(1), dim3(kIntThreads), lds, s, p, it);
             SARF_TRY(k1_row_step(p, 1, s));
             SARF_HIP(hipGetLastError());
         }
@@ -944,8 +971,7 @@ extern "C" int sarf_pga_scenes(sarf_ctx *ctx, int nscenes, const sarf_pga_scene 
     for (int i = 0; i < n; i++) {
         const PgaImage &p = host[i];
         const int cnt[kOffKinds] = {((p.Nx + 63) / 64) * ((p.Ny + 63) / 64), p.Ny,
-                                    ((p.Nx + 63) / 64) * kRowChunks,
-                                    (p.Nx + kThreads - 1) / kThreads};
+                                    ((p.Nx + 63) / 64) * kRowChunks};
         for (int k = 0; k < kOffKinds; k++)
             off[(size_t)k * (n + 1) + i + 1] = off[(size_t)k * (n + 1) + i] + cnt[k];
     }
@@ -1019,17 +1045,13 @@ extern "C" int sarf_pga_scenes(sarf_ctx *ctx, int nscenes, const sarf_pga_scene 
     if (iters > 0) {
         hipLaunchKernelGGL(kb_transpose, dim3(total(kOffTranspose)), dim3(kThreads), 0, s, b, 1);
         for (int it = 0; it < iters; it++) {
-            hipLaunchKernelGGL(kb_argmax, dim3(total(kOffRows)), dim3(kThreads), 0, s, b);
-            hipLaunchKernelGGL(kb_col_partial, dim3(total(kOffColPartial)), dim3(kThreads), 0, s,
-                               b, 0);
-            hipLaunchKernelGGL(kb_col_final, dim3(total(kOffColFinal)), dim3(kThreads), 0, s, b, 0);
+            hipLaunchKernelGGL(kb_stats, dim3(total(kOffRows) + total(kOffColPartial)),
+                               dim3(kThreads), 0, s, b, total(kOffRows));
             hipLaunchKernelGGL(kb_window, dim3(n), dim3(kIntThreads), 0, s, b, db_down);
             for (const Group &g : groups) SARF_TRY(kb_row_step(b, g.L, g.g0, g.m, g.blocks, 0, s));
             hipLaunchKernelGGL(kb_col_partial, dim3(total(kOffColPartial)), dim3(kThreads), 0, s,
                                b, 1);
-            hipLaunchKernelGGL(kb_col_final, dim3(total(kOffColFinal)), dim3(kThreads), 0, s, b, 1);
-            hipLaunchKernelGGL(kb_integrate, dim3(n), dim3(kIntThreads), 0, s, b, it);
-            hipLaunchKernelGGL(kb_apply_spectrum, dim3(n), dim3(kThreads), lds, s, b);
+            hipLaunchKernelGGL(kb_integrate, dim3(n), dim3(kIntThreads), lds, s, b, it);
             for (const Group &g : groups) SARF_TRY(kb_row_step(b, g.L, g.g0, g.m, g.blocks, 1, s));
             SARF_HIP(hipGetLastError());
         }
