@@ -99,9 +99,9 @@ typedef struct {
     int K;     /* frequency samples per pulse, 1 <= K <= Nfft */
     int Np;    /* pulses, >= 1 */
     int Nfft;  /* any length >= K: powers of two up to 2^24 (up to 16384 in one LDS
-                  pass, longer through global-memory passes); other lengths up to 8192
-                  by Bluestein's identity, longer ones (and FP64) by a direct DFT
-                  (O(K) per bin, at most 2^20) */
+                  pass, longer through global-memory passes); other lengths up to 2^20
+                  by Bluestein's identity (up to 8192 register-resident in one kernel,
+                  longer ones and FP64 through the global-memory passes) */
     int flags; /* SARF_RC_* */
 } sarf_rc_desc;
 

@@ -67,26 +67,39 @@ def test_rc_f64_matches_oracle(ctx, nfft):
     assert oracle.rel_l2(prof, oracle.range_compress_f64(ph, nfft)) < 1e-12
 
 
-@pytest.mark.parametrize("nfft", [3, 37, 424, 1000, 3000, 8191, 8192 + 57])
+def _rc_ref(ph, nfft):
+    """FP64 range compression of ph: the oracle up to 16384 bins; above, the
+    numpy expression the oracle is pinned to in test_oracle.py
+    (roll(ifft(ph, N), -((N+1)//2)), i.e. cshift((N+1)/2)), since the oracle's
+    direct DFT of a length that is not a power of two costs O(N K)."""
+    if nfft <= 16384:
+        return oracle.range_compress_f64(ph, nfft)
+    x = np.fft.ifft(np.asarray(ph, np.complex128), nfft, axis=1)
+    return np.roll(x, -((nfft + 1) // 2), axis=1)
+
+
+@pytest.mark.parametrize("nfft", [3, 37, 424, 1000, 3000, 8191, 8192 + 57, 30011, 100003])
 def test_rc_non_power_of_two_matches_oracle(ctx, nfft):
     """Nfft that is not a power of two (N_fft = K is the reference's default,
-    charlotte_sar_api.hpp:173): Bluestein's identity up to 8192 (odd lengths
-    included: the fftshift remap), a direct DFT above; FP32 <= 2e-6 relative
-    L2 against the FP64 oracle on the same complex64 input, with taper and
-    ramp on one case; FP64 (direct) <= 1e-12."""
+    charlotte_sar_api.hpp:173): Bluestein's identity, register-resident up
+    to 8192 (odd lengths included: the fftshift remap) and through the
+    global-memory passes beyond (and in FP64); FP32 <= 2e-6 relative L2
+    against the FP64 reference on the same complex64 input, with taper and
+    ramp on one case; FP64 <= 1e-12."""
     rng = np.random.default_rng(nfft)
     Np = 5
     for K in sorted({1, min(nfft, 424), nfft}):
         ph = rng.standard_normal((Np, K)) + 1j * rng.standard_normal((Np, K))
         p32 = ctx.range_compress(to_dev(ph.astype(np.complex64)), nfft).cpu().numpy()
-        e = oracle.rel_l2(p32, oracle.range_compress_f64(ph.astype(np.complex64), nfft))
+        e = oracle.rel_l2(p32, _rc_ref(ph.astype(np.complex64), nfft))
         assert e < 2e-6, (nfft, K, e)
     p64 = ctx.range_compress(to_dev(ph), nfft).cpu().numpy()
-    assert oracle.rel_l2(p64, oracle.range_compress_f64(ph, nfft)) < 1e-12
-    taper = np.hanning(K).astype(np.float32)
+    assert oracle.rel_l2(p64, _rc_ref(ph, nfft)) < 1e-12
+    Kt = min(nfft, 424)
+    ph32 = (rng.standard_normal((Np, Kt)) + 1j * rng.standard_normal((Np, Kt))).astype(np.complex64)
+    taper = np.hanning(Kt).astype(np.float32)
     f0 = 9.6e9 + 1e6 * np.arange(Np)
     r0 = 12500.0 + 0.37 * np.arange(Np)
-    ph32 = ph.astype(np.complex64)
     pt = ctx.range_compress(to_dev(ph32), nfft, taper=to_dev(taper),
                             ramp=(to_dev(f0), to_dev(r0))).cpu().numpy()
     ref = oracle.range_compress_f64(ph32, nfft, taper=taper, ramp=(f0, r0))
@@ -112,7 +125,7 @@ def test_rc_rejects_bad_sizes(ctx):
         ctx.range_compress(ph, 8)  # K > Nfft
     assert e.value.code == _lib.SARF_ERR_INVALID
     with pytest.raises(_lib.SarfError) as e:
-        ctx.range_compress(ph, (1 << 20) + 1)  # beyond the direct transform
+        ctx.range_compress(ph, (1 << 20) + 1)  # beyond the Bluestein range
     assert e.value.code == _lib.SARF_ERR_UNSUPPORTED
 
 

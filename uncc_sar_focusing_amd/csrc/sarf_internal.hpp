@@ -59,8 +59,9 @@ struct sarf_ctx {
     // DFT twiddle tables e^{+j 2 pi i / N}, keyed by N
     std::map<long, float2 *> tw32;
     std::map<long, double2 *> tw64;
-    // Bluestein range-compression tables, keyed by the transform length
-    std::map<long, float2 *> rc_blue;
+    // Bluestein range-compression tables, keyed by the transform length N
+    // (FP32 tables) or -N (FP64 tables)
+    std::map<long, void *> rc_blue;
     // named scratch buffers (partial images, reductions, staging); a buffer
     // that has to grow is retired, not freed, so work enqueued earlier --
     // on any stream, or in a captured graph -- never sees freed memory

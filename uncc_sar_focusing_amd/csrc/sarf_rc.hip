@@ -17,8 +17,10 @@
 //   * taper, 1/N and the ramp, on the first load / last store.
 // LDS rows are padded by one element every 16 so the stride-16 scatter of
 // the first pass is bank-conflict free.  N up to 16384 fits (139 KiB of the
-// 160 KiB LDS).  The FP64 variant (validation builds) uses a global-memory
-// radix-4 Stockham since 16384 double2 do not fit in LDS.
+// 160 KiB LDS).  The FP64 variant (validation builds) and FP32 powers of two
+// outside 32..16384 use a global-memory radix-4 Stockham; lengths that are
+// not a power of two use Bluestein's identity (register-resident up to 8192,
+// through the global-memory passes beyond).
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -36,8 +38,11 @@ using fft::dft;
 using fft::lpad;
 using fft::powers16;
 
-// Longest non-power-of-two transform done by Bluestein (M <= 16384 in LDS).
+// Longest non-power-of-two transform done by the register-resident
+// Bluestein kernel (M <= 16384); longer ones (and FP64) go through the
+// global-memory passes, up to 2^20 (M = 2^21).
 constexpr int kBluesteinMaxN = 8192;
+constexpr int kBluesteinGlobalMaxN = 1 << 20;
 
 constexpr int rc_threads(int log2n) {
     return ((1 << log2n) / 16 >= 256) ? (1 << log2n) / 16 : 256;
@@ -328,62 +333,72 @@ __global__ __launch_bounds__(256) void rc_pass(
 }
 
 // ---------------------------------------------------------------------------
-// Any Nfft (not a power of two, e.g. the reference's default N_fft = K,
-// charlotte_sar_api.hpp:173): direct inverse DFT, O(K) per output bin.
-//   prof[m] = (1/N) sum_{k<K} x_k e^{+j 2 pi k m' / N},  m' = (m + (N+1)/2) mod N
-// The twiddle index k m' mod N is carried exactly in integers and looked up
-// in the e^{+j 2 pi i / N} table, so no phase error accumulates.  A workgroup
-// stages one pulse's K samples in LDS (chunks of 2048) and each lane forms
-// one output bin.
+// Nfft not a power of two beyond the register-resident range (FP32 above
+// 8192, every FP64 length): the same Bluestein identity as rc_bluestein below,
+// with the two length-M transforms done by the global-memory passes.  Per
+// pulse, in rows of M:
+//   rcb_pre : z_k = conj(x_k taper_k c_k) for k < K, 0 up to M
+//   passes  : Z = IFFT_M(z) unscaled = conj(FFT_M(x c))
+//   rcb_mul : U = conj(Z) .* C                         (C = FFT_M(h), h the
+//             circularly extended conj(c); FFT_M(x c) .* C = FFT_M((x c) * h))
+//   passes  : W = IFFT_M(U) unscaled = M ((x c) * h)
+//   rcb_post: X_m = c_m W_m / (M N) (ramp), stored at the fftshift position.
 // ---------------------------------------------------------------------------
-template <typename T2, typename R>
-__global__ __launch_bounds__(256) void rc_direct(const T2 *__restrict__ ph, int K, int N,
-                                                 int flags, const float *__restrict__ taper,
-                                                 const double *__restrict__ f0,
-                                                 const double *__restrict__ r0,
-                                                 const T2 *__restrict__ tw, T2 *__restrict__ prof) {
-    constexpr int kChunk = 2048;
-    __shared__ T2 xs[kChunk];
-    const int n = blockIdx.y;
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const int mp = (int)(((long)m + (N + 1) / 2) % N);
-    R ar = 0, ai = 0;
-    for (int k0 = 0; k0 < K; k0 += kChunk) {
-        const int kn = min(kChunk, K - k0);
-        __syncthreads();
-        for (int i = threadIdx.x; i < kn; i += blockDim.x) {
-            T2 x = ph[(size_t)n * K + k0 + i];
-            if (flags & SARF_RC_TAPER) {
-                x.x *= taper[k0 + i];
-                x.y *= taper[k0 + i];
-            }
-            xs[i] = x;
+template <typename T2>
+__global__ __launch_bounds__(256) void rcb_pre(const T2 *__restrict__ ph, int K, int M, long total,
+                                               int flags, const float *__restrict__ taper,
+                                               const T2 *__restrict__ chirp, T2 *__restrict__ z) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    const long n = gid / M;
+    const int k = (int)(gid - n * M);
+    T2 v{};
+    if (k < K) {
+        T2 x = ph[n * K + k];
+        if (flags & SARF_RC_TAPER) {
+            x.x *= taper[k];
+            x.y *= taper[k];
         }
-        __syncthreads();
-        if (m < N) {
-            long idx = ((long)k0 * mp) % N;  // (k m') mod N for k = k0
-            for (int i = 0; i < kn; i++) {
-                const T2 w = tw[idx];
-                const T2 x = xs[i];
-                ar += (R)x.x * w.x - (R)x.y * w.y;
-                ai += (R)x.x * w.y + (R)x.y * w.x;
-                idx += mp;
-                if (idx >= N) idx -= N;
-            }
-        }
+        v = cmul(x, chirp[k]);
+        v.y = -v.y;
     }
-    if (m < N) {
-        double sr = 1.0 / N, si = 0.0;
-        if (flags & SARF_RC_RAMP) {
-            const double a = -4.0 * kPi * f0[n] * r0[n] / kClight;
-            sr = cos(a) / N;
-            si = sin(a) / N;
-        }
-        T2 o;
-        o.x = (decltype(o.x))(ar * (R)sr - ai * (R)si);
-        o.y = (decltype(o.y))(ar * (R)si + ai * (R)sr);
-        prof[(size_t)n * N + m] = o;
+    z[gid] = v;
+}
+
+template <typename T2>
+__global__ __launch_bounds__(256) void rcb_mul(T2 *__restrict__ z, int M, long total,
+                                               const T2 *__restrict__ C) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    T2 v = z[gid];
+    v.y = -v.y;
+    z[gid] = cmul(v, C[gid % M]);
+}
+
+template <typename T2>
+__global__ __launch_bounds__(256) void rcb_post(const T2 *__restrict__ w, int M, int N, long total,
+                                                int flags, const double *__restrict__ f0,
+                                                const double *__restrict__ r0,
+                                                const T2 *__restrict__ chirp, T2 *__restrict__ prof) {
+    using Re = decltype(T2{}.x);
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    const long n = gid / N;
+    const int m = (int)(gid - n * N);
+    double sr = 1.0 / ((double)M * N), si = 0.0;
+    if (flags & SARF_RC_RAMP) {
+        const double a = -4.0 * kPi * f0[n] * r0[n] / kClight;
+        sr = cos(a) / ((double)M * N);
+        si = sin(a) / ((double)M * N);
     }
+    const T2 X = cmul(w[n * M + m], chirp[m]);
+    const int half = (N + 1) / 2;
+    int o = m - half;
+    if (o < 0) o += N;
+    T2 y;
+    y.x = X.x * (Re)sr - X.y * (Re)si;
+    y.y = X.x * (Re)si + X.y * (Re)sr;
+    prof[n * N + o] = y;
 }
 
 // ---------------------------------------------------------------------------
@@ -529,15 +544,17 @@ int launch_jobs(const sarf_ctx *ctx, int rows, const RcJob *jobs, int njobs, int
 }
 
 // Bluestein tables of a length-N inverse DFT with transform length M = 2^L
-// (cached per N in the context): chirp[k] = e^{+j pi k^2 / N}, k < N, and
-// C = FFT_M(h) with h_d = conj(chirp[d]) for d < N, h_{M-d} = conj(chirp[d])
-// for 0 < d < N, zero between; evaluated in FP64 (the chirp phase from the
-// exact k^2 mod 2N, the FFT with directly evaluated twiddles) and rounded
-// once.  One allocation [chirp (N) | C (M)].
-int bluestein_tables(sarf_ctx *ctx, int N, int L, const float2 **chirp, const float2 **C) {
+// (cached per N and precision in the context): chirp[k] = e^{+j pi k^2 / N},
+// k < N, and C = FFT_M(h) with h_d = conj(chirp[d]) for d < N,
+// h_{M-d} = conj(chirp[d]) for 0 < d < N, zero between; evaluated in FP64 (the
+// chirp phase from the exact k^2 mod 2N, the FFT with directly evaluated
+// twiddles) and rounded once to T2.  One allocation [chirp (N) | C (M)].
+template <typename T2>
+int bluestein_tables(sarf_ctx *ctx, int N, int L, const T2 **chirp, const T2 **C) {
     const int M = 1 << L;
+    const long key = sizeof(T2) == sizeof(double2) ? -(long)N : (long)N;
     std::lock_guard<std::mutex> lk(ctx->mu);
-    auto it = ctx->rc_blue.find(N);
+    auto it = ctx->rc_blue.find(key);
     if (it == ctx->rc_blue.end()) {
         std::vector<double> cr(N), ci(N);
         for (int k = 0; k < N; k++) {
@@ -555,7 +572,14 @@ int bluestein_tables(sarf_ctx *ctx, int N, int L, const float2 **chirp, const fl
                 hi[M - d] = -ci[d];
             }
         }
-        // forward FFT (e^{-j 2 pi t / M}), iterative radix 2
+        // forward FFT (e^{-j 2 pi t / M}), iterative radix 2 over a table of
+        // the M/2 twiddles, each evaluated directly
+        std::vector<double> wr(M / 2 > 0 ? M / 2 : 1), wi(M / 2 > 0 ? M / 2 : 1);
+        for (int t = 0; t < M / 2; t++) {
+            const double a = -2.0 * kPi * (double)t / (double)M;
+            wr[t] = std::cos(a);
+            wi[t] = std::sin(a);
+        }
         for (int i = 1, r = 0; i < M; i++) {
             int bit = M >> 1;
             for (; r & bit; bit >>= 1) r ^= bit;
@@ -569,32 +593,38 @@ int bluestein_tables(sarf_ctx *ctx, int N, int L, const float2 **chirp, const fl
             const int half = len / 2, step = M / len;
             for (int st = 0; st < M; st += len)
                 for (int q = 0; q < half; q++) {
-                    const double a = -2.0 * kPi * (double)(q * step) / (double)M;
-                    const double wr = std::cos(a), wi = std::sin(a);
+                    const double xr = wr[q * step], xi = wi[q * step];
                     const int i0 = st + q, i1 = i0 + half;
-                    const double tr = hr[i1] * wr - hi[i1] * wi, ti = hr[i1] * wi + hi[i1] * wr;
+                    const double tr = hr[i1] * xr - hi[i1] * xi, ti = hr[i1] * xi + hi[i1] * xr;
                     hr[i1] = hr[i0] - tr;
                     hi[i1] = hi[i0] - ti;
                     hr[i0] += tr;
                     hi[i0] += ti;
                 }
         }
-        std::vector<float2> h((size_t)N + M);
-        for (int k = 0; k < N; k++) h[k] = make_float2((float)cr[k], (float)ci[k]);
-        for (int t = 0; t < M; t++) h[(size_t)N + t] = make_float2((float)hr[t], (float)hi[t]);
-        float2 *dv = nullptr;
-        hipError_t e = hipMalloc(&dv, sizeof(float2) * h.size());
+        using Re = decltype(T2{}.x);
+        std::vector<T2> h((size_t)N + M);
+        for (int k = 0; k < N; k++) {
+            h[k].x = (Re)cr[k];
+            h[k].y = (Re)ci[k];
+        }
+        for (int t = 0; t < M; t++) {
+            h[(size_t)N + t].x = (Re)hr[t];
+            h[(size_t)N + t].y = (Re)hi[t];
+        }
+        void *dv = nullptr;
+        hipError_t e = hipMalloc(&dv, sizeof(T2) * h.size());
         if (e != hipSuccess)
             return fail(SARF_ERR_NOMEM, "range_compress: Bluestein tables: %s", hipGetErrorString(e));
-        e = hipMemcpy(dv, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice);
+        e = hipMemcpy(dv, h.data(), sizeof(T2) * h.size(), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             (void)hipFree(dv);
             return fail(SARF_ERR_HIP, "range_compress: Bluestein tables: %s", hipGetErrorString(e));
         }
-        it = ctx->rc_blue.emplace(N, dv).first;
+        it = ctx->rc_blue.emplace(key, dv).first;
     }
-    *chirp = it->second;
-    *C = it->second + N;
+    *chirp = (const T2 *)it->second;
+    *C = (const T2 *)it->second + N;
     return SARF_OK;
 }
 
@@ -643,6 +673,56 @@ int launch_passes(const sarf_rc_desc *d, int log2n, const T2 *ph, const float *t
     return SARF_OK;
 }
 
+// Unscaled inverse transforms of length M = 2^L of Np rows held in x,
+// ping-ponging with y; returns the buffer holding the result.
+template <typename T2>
+T2 *inverse_passes(T2 *x, T2 *y, int L, int Np, const T2 *tw, hipStream_t s) {
+    const int M = 1 << L;
+    T2 *in = x, *out = y;
+    int Ns = 1;
+    if (L & 1) {
+        const long threads = (long)Np * (M / 2);
+        hipLaunchKernelGGL((rc_pass<T2, 2>), dim3((threads + 255) / 256), dim3(256), 0, s, in, out,
+                           M, Ns, Np, tw, 0, M, 0, nullptr, 0, nullptr, nullptr);
+        std::swap(in, out);
+        Ns *= 2;
+    }
+    for (; Ns < M; Ns *= 4) {
+        const long threads = (long)Np * (M / 4);
+        hipLaunchKernelGGL((rc_pass<T2, 4>), dim3((threads + 255) / 256), dim3(256), 0, s, in, out,
+                           M, Ns, Np, tw, 0, M, 0, nullptr, 0, nullptr, nullptr);
+        std::swap(in, out);
+    }
+    return in;
+}
+
+// Bluestein through global memory (rcb_pre / passes / rcb_mul / passes /
+// rcb_post), two scratch rows of M per pulse.
+template <typename T2>
+int launch_bluestein_global(sarf_ctx *ctx, const sarf_rc_desc *d, int L, const T2 *ph,
+                            const float *taper, const double *f0, const double *r0,
+                            const T2 *tw, T2 *prof, hipStream_t s) {
+    const int N = d->Nfft, M = 1 << L;
+    const T2 *chirp = nullptr, *C = nullptr;
+    SARF_TRY(bluestein_tables<T2>(ctx, N, L, &chirp, &C));
+    const long rows = (long)d->Np * M;
+    void *a = nullptr, *b = nullptr;
+    SARF_TRY(scratch_get(ctx, "rcb.a", sizeof(T2) * (size_t)rows, &a));
+    SARF_TRY(scratch_get(ctx, "rcb.b", sizeof(T2) * (size_t)rows, &b));
+    T2 *za = (T2 *)a, *zb = (T2 *)b;
+    const dim3 gm((unsigned)((rows + 255) / 256));
+    hipLaunchKernelGGL(rcb_pre<T2>, gm, dim3(256), 0, s, ph, d->K, M, rows, d->flags, taper,
+                       chirp, za);
+    T2 *z = inverse_passes<T2>(za, zb, L, d->Np, tw, s);
+    hipLaunchKernelGGL(rcb_mul<T2>, gm, dim3(256), 0, s, z, M, rows, C);
+    T2 *w = inverse_passes<T2>(z, z == za ? zb : za, L, d->Np, tw, s);
+    const long outs = (long)d->Np * N;
+    hipLaunchKernelGGL(rcb_post<T2>, dim3((unsigned)((outs + 255) / 256)), dim3(256), 0, s, w, M, N,
+                       outs, d->flags, f0, r0, chirp, prof);
+    SARF_HIP(hipGetLastError());
+    return SARF_OK;
+}
+
 int rc_pulses_per_group(int log2n) {
     const int T = (1 << log2n) / 16;
     return T >= 256 ? 1 : 256 / T;
@@ -668,7 +748,7 @@ int launch_range_compress(sarf_ctx *ctx, const sarf_rc_desc *d, const void *ph,
         int L = 5;
         while ((1 << L) < 2 * d->Nfft - 1) L++;
         const float2 *chirp = nullptr, *C = nullptr, *tw = nullptr;
-        SARF_TRY(bluestein_tables(ctx, d->Nfft, L, &chirp, &C));
+        SARF_TRY(bluestein_tables<float2>(ctx, d->Nfft, L, &chirp, &C));
         SARF_TRY(twiddles_f32(ctx, L, &tw));
         const float2 *x = (const float2 *)ph;
         float2 *y = (float2 *)prof;
@@ -689,27 +769,23 @@ int launch_range_compress(sarf_ctx *ctx, const sarf_rc_desc *d, const void *ph,
         return SARF_OK;
     }
     if (log2n < 0) {
-        // not a power of two, beyond the Bluestein range (or FP64): direct
-        // DFT with an exact twiddle index
-        if (d->Nfft > (1 << 20) || d->Np > 65535)
-            return fail(SARF_ERR_UNSUPPORTED, "range_compress: Nfft=%d / Np=%d too large for the "
-                        "direct transform", d->Nfft, d->Np);
-        dim3 grid((d->Nfft + 255) / 256, d->Np);
+        // not a power of two, beyond the register-resident range (or FP64):
+        // Bluestein through the global-memory passes
+        if (d->Nfft > kBluesteinGlobalMaxN)
+            return fail(SARF_ERR_UNSUPPORTED, "range_compress: Nfft=%d (lengths that are not a "
+                        "power of two go up to %d)", d->Nfft, kBluesteinGlobalMaxN);
+        int L = 1;
+        while ((1 << L) < 2 * d->Nfft - 1) L++;
         if (f64) {
             const double2 *tw = nullptr;
-            SARF_TRY(twiddles_any_f64(ctx, d->Nfft, &tw));
-            hipLaunchKernelGGL((rc_direct<double2, double>), grid, dim3(256), 0, s,
-                               (const double2 *)ph, d->K, d->Nfft, d->flags, taper, f0, r0, tw,
-                               (double2 *)prof);
-        } else {
-            const float2 *tw = nullptr;
-            SARF_TRY(twiddles_any_f32(ctx, d->Nfft, &tw));
-            hipLaunchKernelGGL((rc_direct<float2, float>), grid, dim3(256), 0, s,
-                               (const float2 *)ph, d->K, d->Nfft, d->flags, taper, f0, r0, tw,
-                               (float2 *)prof);
+            SARF_TRY(twiddles_f64(ctx, L, &tw));
+            return launch_bluestein_global<double2>(ctx, d, L, (const double2 *)ph, taper, f0, r0,
+                                                    tw, (double2 *)prof, s);
         }
-        SARF_HIP(hipGetLastError());
-        return SARF_OK;
+        const float2 *tw = nullptr;
+        SARF_TRY(twiddles_f32(ctx, L, &tw));
+        return launch_bluestein_global<float2>(ctx, d, L, (const float2 *)ph, taper, f0, r0, tw,
+                                               (float2 *)prof, s);
     }
     if (f64 || log2n > 14 || log2n < 5) {
         if (log2n < 1 || log2n > 24)
