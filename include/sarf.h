@@ -159,8 +159,11 @@ typedef struct {
     int tile;       /* 0 = default for the pixel spacing and line of sight; else a
                        tile id: 1..7 (lanes along y) or 11..14 (lanes along x);
                        batches and the gradient take 1..5 and 11..14 */
-    int pulse_split;/* 0 = automatic; n >= 1: pulses in n contiguous groups, each
-                       forming a partial image, summed in group order */
+    int pulse_split;/* 0 = automatic; n >= 1: pulses in n contiguous groups summed
+                       in group order -- n even groups, each a partial image, or,
+                       when the image's tiles fill two rounds of resident
+                       workgroups, one head group of the first 7/8 of the pulses
+                       written into img and n - 1 tail groups as partial images */
     double los_x, los_y; /* optional hint: horizontal components of the unit line
                             of sight from the image centre to the aperture centre;
                             (0,0) = unknown.  Only steers the default tile choice. */
@@ -178,8 +181,9 @@ int sarf_backproject(sarf_ctx *ctx, const sarf_bp_desc *desc,
 /* is allocated or freed in the call (no hipMalloc/hipFree, no context      */
 /* scratch).  Partial images of pulse splits go to a 64 MiB device array    */
 /* that the library holds per GPU for the life of the process: an automatic */
-/* or tuned split count is lowered until Nx*Ny*8*splits fits, and an        */
-/* explicit desc->pulse_split that does not fit returns SARF_ERR_INVALID.   */
+/* or tuned split count is lowered until its partial images (Nx*Ny*8 bytes  */
+/* each, one per group but the head group) fit, and an explicit             */
+/* desc->pulse_split that does not fit returns SARF_ERR_INVALID.            */
 /* Every call on one GPU shares that array: calls that overlap on a GPU     */
 /* (other streams, other threads, other contexts) race.  No counterpart in  */
 /* the reference; it is for callers that cannot allocate -- code under HIP  */

@@ -13,7 +13,7 @@ import torch
 import oracle
 from scene_data import c0_inputs, local_peaks, peak_index, small_inputs
 from uncc_sar_focusing_amd import _lib
-from uncc_sar_focusing_amd.scenes import ImageGrid, circular_aperture
+from uncc_sar_focusing_amd.scenes import ImageGrid, circular_aperture, point_targets
 
 pytestmark = pytest.mark.gpu
 
@@ -415,6 +415,37 @@ def test_bp_pulse_split_partials_match_oracle(ctx, split):
     # accumulate on top of an existing image
     base = to_dev(np.full((grid.Nx, grid.Ny), 1 + 2j, np.complex64))
     ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr, pulse_split=split, out=base,
+                    accumulate=True)
+    assert oracle.rel_l2(base.cpu().numpy() - (1 + 2j), ref) <= 1e-4
+
+
+@pytest.mark.parametrize("split", [0, 2, 5])
+def test_bp_head_tail_split_matches_oracle(ctx, split):
+    """An image whose tiles fill two rounds of resident workgroups (2048^2
+    with tile 13: 2048 tiles): one head group of the first 7/8 of the pulses
+    writes the image directly and split - 1 tail groups add partial images
+    (0 = the automatic count, 3).  Within 1e-4 rel-L2 of the FP64 oracle
+    with the same brightest pixel, also when accumulating onto an existing
+    image, and within 2e-6 of the unsplit image."""
+    Np, K, n, sp, nfft = 128, 256, 2048, 0.05, 2048
+    geom = circular_aperture(Np, K)
+    tg = point_targets(5, 0.6 * sp * n, seed=9)
+    ph = oracle.synth_ph_f64(geom, tg).astype(np.complex64)
+    grid = ImageGrid.square(n, sp)
+    ant, r0, f0 = geom_dev(geom)
+    prof = ctx.range_compress(to_dev(ph), nfft)
+    img = ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr, tile=13, pulse_split=split)
+    one = ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr, tile=13, pulse_split=1)
+    got = img.cpu().numpy()
+    assert oracle.rel_l2(got, one.cpu().numpy()) <= 2e-6
+    ref = oracle.bp_f64(oracle.range_compress_f64(ph, nfft), geom.ant, geom.r0, geom.f0, grid,
+                        geom.maxWr)
+    e = oracle.rel_l2(got, ref)
+    print(f"head+tail split={split}: rel-L2 vs oracle {e:.3e}", flush=True)
+    assert e <= 1e-4
+    assert peak_index(got) == peak_index(ref)
+    base = to_dev(np.full((grid.Nx, grid.Ny), 1 + 2j, np.complex64))
+    ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr, tile=13, pulse_split=split, out=base,
                     accumulate=True)
     assert oracle.rel_l2(base.cpu().numpy() - (1 + 2j), ref) <= 1e-4
 

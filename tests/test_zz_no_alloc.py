@@ -63,8 +63,11 @@ def test_noalloc_matches_backproject_two_calls(ctx, tile, split):
 
 
 def test_noalloc_lowers_automatic_splits_to_fit(ctx):
-    """2048^2 image: 64 MiB holds two 32 MiB partial images, so the
-    automatic split count (8 here) is lowered to 2."""
+    """2048^2 image (2048 tiles of tile 13: head + tail splits): the
+    automatic 3 groups write 2 partial images of 32 MiB, which fit the 64 MiB
+    as they are; 2560^2: 2 partials of 50 MiB do not, so the allocation-free
+    entry lowers the count to 2 groups (one partial) and matches
+    sarf_backproject with 2 bit for bit."""
     lib = ctx.lib
     geom, ph, ant, r0, f0, grid, nfft = _scene(256, 256, 2048, 0.05, 2048, 5)
     prof = ctx.range_compress(ph, nfft)
@@ -73,12 +76,28 @@ def test_noalloc_lowers_automatic_splits_to_fit(ctx):
     rc, got = _call(ctx, lib.sarf_backproject_noalloc, prof, ant, r0, f0, grid, geom.maxWr, 13, 0,
                     los)
     assert rc == 0, lib.sarf_last_error()
-    rc, two = _call(ctx, lib.sarf_backproject, prof, ant, r0, f0, grid, geom.maxWr, 13, 2, los)
-    assert rc == 0
-    assert np.array_equal(got.view(np.uint32), two.view(np.uint32))
     rc, auto = _call(ctx, lib.sarf_backproject, prof, ant, r0, f0, grid, geom.maxWr, 13, 0, los)
     assert rc == 0
-    assert oracle.rel_l2(got, auto) < 1e-6
+    assert np.array_equal(got.view(np.uint32), auto.view(np.uint32))
+    rc, three = _call(ctx, lib.sarf_backproject, prof, ant, r0, f0, grid, geom.maxWr, 13, 3, los)
+    assert rc == 0
+    assert np.array_equal(got.view(np.uint32), three.view(np.uint32))
+    print("noalloc case 2560^2 automatic shape (lowered)", flush=True)
+    geom5, ph5, ant5, r05, f05, grid5, _ = _scene(256, 256, 2560, 0.05, 2048, 7)
+    prof5 = ctx.range_compress(ph5, nfft)
+    los5 = geom5.los()
+    rc, low = _call(ctx, lib.sarf_backproject_noalloc, prof5, ant5, r05, f05, grid5, geom5.maxWr,
+                    13, 0, los5)
+    assert rc == 0, lib.sarf_last_error()
+    rc, two = _call(ctx, lib.sarf_backproject, prof5, ant5, r05, f05, grid5, geom5.maxWr, 13, 2,
+                    los5)
+    assert rc == 0
+    assert np.array_equal(low.view(np.uint32), two.view(np.uint32))
+    rc, auto5 = _call(ctx, lib.sarf_backproject, prof5, ant5, r05, f05, grid5, geom5.maxWr, 13, 0,
+                      los5)
+    assert rc == 0
+    assert oracle.rel_l2(low, auto5) < 1e-6
+    del prof5, ph5
     # an explicit split count whose partial images exceed the scratch
     rc, _ = _call(ctx, lib.sarf_backproject_noalloc, prof, ant, r0, f0, grid, geom.maxWr, 13, 8,
                   los)

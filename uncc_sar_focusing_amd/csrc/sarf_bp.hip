@@ -25,14 +25,18 @@ void bp_mixed(BpArgs a, const float2 *__restrict__ prof,
     __shared__ BpLds L;
     __shared__ __align__(16) float fx[BpTile<TY, PX, XL>::kFx];
     // blockIdx.x = split * ntiles + tile: every split of a tile covers a
-    // contiguous pulse range and writes its own partial image
+    // contiguous pulse range and writes its own partial image -- except the
+    // head group of a head + tail split, which writes the image itself
     const int ntiles = a.tiles_x * a.tiles_y;
     const int split = blockIdx.x / ntiles;
     const int t = swizzle_tile(blockIdx.x - split * ntiles, ntiles);
-    const int per_split = (a.Np + a.splits - 1) / a.splits;
-    const int nbeg = min(a.Np, split * per_split);
-    const int nend = min(a.Np, nbeg + per_split);
-    img += (size_t)split * a.split_stride;
+    int nbeg, nend;
+    split_range(a, split, nbeg, nend);
+    int accumulate = a.accumulate;
+    if (a.splits > 1 && !(a.tail > 0 && split == 0)) {
+        img = a.part + (size_t)(a.tail > 0 ? split - 1 : split) * a.split_stride;
+        accumulate = 0;
+    }
 
     BpTile<TY, PX, XL> T;
     T.init(a, t);
@@ -42,7 +46,7 @@ void bp_mixed(BpArgs a, const float2 *__restrict__ prof,
         int x, y;
         if (T.pixel(j, x, y, a)) {
             float2 *o = img + (size_t)x * a.Ny + y;
-            if (a.accumulate) {
+            if (accumulate) {
                 const float2 old = *o;
                 *o = make_float2(old.x + T.accr[j], old.y + T.acci[j]);
             } else {
@@ -203,6 +207,14 @@ int bp_tile_shape(int id, int *tx, int *ty) {
     return 0;
 }
 
+// Waves per SIMD the tile's kernel is built for (= resident workgroups per
+// CU: 256-thread workgroups, 4 SIMDs); 1 for an unknown id.
+int tile_waves_per_eu(int tile) {
+    for (const TileShape &t : kTiles)
+        if (t.id == tile) return bp_waves_per_eu_rt(t.tl, t.px, t.xl != 0);
+    return 1;
+}
+
 // Validates a mixed/F64 descriptor and fills the kernel arguments for tile
 // `tile` (0 = default choice).  Splits are left at 1.
 int bp_fill_args(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, BpArgs *out) {
@@ -227,6 +239,7 @@ int bp_fill_args(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, BpArgs *out) {
     a.maxWr = d->maxWr;
     a.invbin = (float)(d->Nfft / d->maxWr);
     a.splits = 1;
+    a.tail = 0;
     a.split_stride = 0;
     if (d->precision == SARF_BP_MIXED) {
         if (tile <= 0) tile = bp_pick_tile(d, a.dx, a.dy);
@@ -276,17 +289,42 @@ std::array<uint64_t, 8> tune_key(const sarf_tune_entry &e) {
 
 std::array<uint64_t, 8> tune_key(const sarf_bp_desc *d) { return tune_key(tune_geometry(d)); }
 
-// Default pulse splits: split the pulses into groups that form partial
-// images (summed in order afterwards) until there are 64 workgroups per CU,
-// keeping at least 16 pulses per group.  Measured on MI355X (tile 13): 2048^2
-// x 4096 pulses best at 8 groups (7.54 ms vs 8.18 unsplit), 512^2 x 1024 at
-// 16-32, 256^2 x 512 at 16.
-int default_splits(const sarf_ctx *ctx, int ntiles, int Np) {
+// Pulse splits.  A tile's pulses are split into groups, each a workgroup of
+// its own writing a partial image, summed in order afterwards.  Two rules:
+// * head + tail, when the tiles alone fill at least two rounds of resident
+//   workgroups: one head group of the first 7/8 of the pulses per tile
+//   writes the image directly, and the last 1/8 is spread over splits - 1
+//   short groups that the split-major launch order runs last, filling in as
+//   CUs free up (default 3 splits).  Measured on MI355X at 2048^2 x 4096
+//   pulses, tile 13 (2048 tiles, 1024 resident): 7.24 ms, against 7.32 ms
+//   for 8 even groups and 8.04 ms unsplit, with 2 partial images instead of 8
+//   (a tail of 1/4, 1/6, 1/12 or 1/16 of the pulses measured 7.27-7.58 ms);
+// * even groups otherwise (few tiles: the groups are what fills the GPU),
+//   doubling until there are 64 workgroups per CU with at least 16 pulses
+//   per group (512^2 x 1024 pulses best at 16-32, 256^2 x 512 at 16).
+
+bool split_head_tail(const sarf_ctx *ctx, int tile, int ntiles, int Np) {
+    const long resident = (long)(ctx->cu_count > 0 ? ctx->cu_count : 256) * tile_waves_per_eu(tile);
+    return (long)ntiles >= 2 * resident && Np >= 64;
+}
+
+int default_splits(const sarf_ctx *ctx, int tile, int ntiles, int Np) {
+    if (split_head_tail(ctx, tile, ntiles, Np)) return 3;
     int splits = 1;
     const long want = 64L * (ctx->cu_count > 0 ? ctx->cu_count : 256);
     while (splits < 32 && (long)ntiles * splits < want && Np / (2 * splits) >= 16) splits *= 2;
     return splits;
 }
+
+// Tail pulses of a launch with `splits` groups (0: even groups).
+int split_tail(const sarf_ctx *ctx, int tile, int ntiles, int Np, int splits) {
+    if (splits < 2 || !split_head_tail(ctx, tile, ntiles, Np)) return 0;
+    const int tail = std::max(splits - 1, Np / 8);
+    return tail < Np ? tail : 0;
+}
+
+// Partial images a launch writes.
+int split_partials(int splits, int tail) { return splits < 2 ? 0 : tail > 0 ? splits - 1 : splits; }
 
 // Mixed-precision launch with a resolved tile id (> 0) and split count
 // (0 = default rule).
@@ -301,27 +339,28 @@ int launch_mixed_shape(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, int split
     int tx = 0, ty = 0;
     const int tid = bp_tile_shape(tile, &tx, &ty);
     const int ntiles = a.tiles_x * a.tiles_y;
-    if (splits <= 0) splits = default_splits(ctx, ntiles, d->Np);
+    if (splits <= 0) splits = default_splits(ctx, tile, ntiles, d->Np);
     splits = std::max(1, std::min(splits, d->Np));
     const long npix = (long)d->Nx * d->Ny;
     float2 *o = (float2 *)img;
-    float2 *part = nullptr;
     a.splits = splits;
+    a.tail = split_tail(ctx, tile, ntiles, d->Np, splits);
     a.split_stride = 0;
-    if (splits > 1) {
+    a.part = nullptr;
+    const int nparts = split_partials(splits, a.tail);
+    if (nparts > 0) {
         void *buf = ext_part;
+        const size_t need = sizeof(float2) * (size_t)npix * nparts;
         if (ext_part) {
-            if (sizeof(float2) * (size_t)npix * splits > ext_bytes)
+            if (need > ext_bytes)
                 return fail(SARF_ERR_INVALID, "backproject: %d pulse splits of a %dx%d image need "
                             "%zu bytes of partial images, more than the %zu available", splits,
-                            d->Nx, d->Ny, sizeof(float2) * (size_t)npix * splits, ext_bytes);
+                            d->Nx, d->Ny, need, ext_bytes);
         } else {
-            SARF_TRY(scratch_get(ctx, "bp.partials", sizeof(float2) * npix * splits, &buf));
+            SARF_TRY(scratch_get(ctx, "bp.partials", need, &buf));
         }
-        part = (float2 *)buf;
-        o = part;
+        a.part = (float2 *)buf;
         a.split_stride = npix;
-        a.accumulate = 0;
     }
     const int grid = ntiles * splits;
     const float2 *p = (const float2 *)prof;
@@ -340,10 +379,12 @@ int launch_mixed_shape(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, int split
         default: return fail(SARF_ERR_INVALID, "backproject: unknown tile id %d", tid);
     }
     SARF_HIP(hipGetLastError());
-    if (splits > 1) {
-        hipLaunchKernelGGL(bp_reduce_splits, dim3((npix + 255) / 256), dim3(256), 0, s, part,
-                           (long)npix, splits, npix, (d->flags & SARF_BP_ACCUMULATE) ? 1 : 0,
-                           (float2 *)img);
+    if (nparts > 0) {
+        // even groups: img (+)= sum of all partials; head + tail: the head
+        // group wrote (or added into) img, the tail partials are added to it
+        const int acc = a.tail > 0 || (d->flags & SARF_BP_ACCUMULATE) ? 1 : 0;
+        hipLaunchKernelGGL(bp_reduce_splits, dim3((npix + 255) / 256), dim3(256), 0, s, a.part,
+                           (long)npix, nparts, npix, acc, (float2 *)img);
         SARF_HIP(hipGetLastError());
     }
     return SARF_OK;
@@ -397,9 +438,12 @@ int launch_backproject_ext(sarf_ctx *ctx, const sarf_bp_desc *d, const void *pro
         if (!bp_tile_shape(tile, &tx, &ty))
             return fail(SARF_ERR_INVALID, "backproject: unknown tile id %d", tile);
         const int ntiles = ((d->Nx + tx - 1) / tx) * ((d->Ny + ty - 1) / ty);
-        if (splits <= 0) splits = default_splits(ctx, ntiles, d->Np);
+        if (splits <= 0) splits = default_splits(ctx, tile, ntiles, d->Np);
         const size_t img_bytes = sizeof(float2) * (size_t)d->Nx * d->Ny;
-        while (splits > 1 && img_bytes * splits > ext_bytes) splits /= 2;
+        while (splits > 1 &&
+               img_bytes * split_partials(splits, split_tail(ctx, tile, ntiles, d->Np, splits)) >
+                   ext_bytes)
+            splits = split_tail(ctx, tile, ntiles, d->Np, splits) > 0 ? splits - 1 : splits / 2;
     }
     return launch_mixed_shape(ctx, d, tile, splits, prof, ant, r0, f0, img, s, ext_part,
                               ext_bytes);
@@ -493,7 +537,7 @@ extern "C" int sarf_autotune(sarf_ctx *ctx, const sarf_bp_desc *d, const void *p
         int tx = 0, ty = 0;
         bp_tile_shape(best_tile, &tx, &ty);
         const int ntiles = ((d->Nx + tx - 1) / tx) * ((d->Ny + ty - 1) / ty);
-        best_split = std::max(1, std::min(default_splits(ctx, ntiles, d->Np), d->Np));
+        best_split = std::max(1, std::min(default_splits(ctx, best_tile, ntiles, d->Np), d->Np));
     }
     // leave the image made with the chosen shape
     SARF_TRY(launch_mixed_shape(ctx, &dd, best_tile, best_split, prof, ant, r0, f0, img, s));

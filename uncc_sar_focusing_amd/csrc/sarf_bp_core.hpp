@@ -63,6 +63,10 @@ template <int TY, int PX, bool XL>
 constexpr int bp_waves_per_eu() {
     return PX > 8 ? 1 : (XL && (kBpThreads / TY) * PX > 32) ? 3 : 4;
 }
+// The same for a tile described at run time (lanes tl, pixels per lane px).
+constexpr int bp_waves_per_eu_rt(int tl, int px, bool xl) {
+    return px > 8 ? 1 : (xl && (kBpThreads / tl) * px > 32) ? 3 : 4;
+}
 constexpr int kBpChunk = 32;  // max pulses staged per chunk
 constexpr int kBpCap = 2048;  // LDS window capacity, in bins (16 B each)
 static_assert(kBpChunk <= 64 && kBpCap % kBpThreads == 0, "staging layout");
@@ -72,11 +76,37 @@ struct BpArgs {
     int tiles_x, tiles_y;
     int accumulate;
     int splits;         // pulse groups per tile (> 1: partial images, reduced after)
+    int tail;           // > 0 (with splits > 1): split 0 takes the first Np - tail
+                        // pulses straight into the image, the other splits share
+                        // the last tail pulses in partial images
     long split_stride;  // elements between partial images
+    float2 *part;       // partial images (splits > 1)
     double left, bottom, dx, dy;  // pixel (i,j) = (left + i dx, bottom + j dy)
     double maxWr;
     float invbin;  // Nfft / maxWr (bins per metre)
 };
+
+// Pulse range [nbeg, nend) of split s of a tile: even groups, or, with a
+// tail, one head group of the first Np - tail pulses and the tail pulses in
+// even groups over the remaining splits.  Split-major launch order runs every
+// tile's head first, so the short tail groups fill in as CUs free up.
+__device__ __forceinline__ void split_range(const BpArgs &a, int s, int &nbeg, int &nend) {
+    if (a.tail > 0 && a.splits > 1) {
+        const int head = a.Np - a.tail;
+        if (s == 0) {
+            nbeg = 0;
+            nend = head;
+            return;
+        }
+        const int per = (a.tail + a.splits - 2) / (a.splits - 1);
+        nbeg = min(a.Np, head + (s - 1) * per);
+        nend = min(a.Np, nbeg + per);
+        return;
+    }
+    const int per = (a.Np + a.splits - 1) / a.splits;
+    nbeg = min(a.Np, s * per);
+    nend = min(a.Np, nbeg + per);
+}
 
 struct BpLds {
     float4 win[kBpCap];
