@@ -139,39 +139,47 @@ struct Shape {
 //   out: v[r]        = X[j + r*T]             (r < 16)
 // Starts with a barrier, so the caller may have just read L; every thread
 // of the workgroup must call it.  The forward transform is conj(ifft(conj)).
+// With active = false a thread only takes part in the barriers (for a
+// workgroup wider than the row's T lanes); its v is left unchanged.
 template <int LOG2N>
 __device__ __forceinline__ void ifft_regs(float2 (&v)[16], float2 *L, int j,
-                                          const float2 *__restrict__ tw) {
+                                          const float2 *__restrict__ tw, bool active = true) {
     using S = Shape<LOG2N>;
     constexpr int N = S::N, T = S::T, R0 = S::R0, NB0 = S::NB0;
     __syncthreads();
+    if (active) {
 #pragma unroll
-    for (int b = 0; b < NB0; b++) {
-        const int jb = j + b * T;
-        dft<+1, R0>(v + b * R0);
+        for (int b = 0; b < NB0; b++) {
+            const int jb = j + b * T;
+            dft<+1, R0>(v + b * R0);
 #pragma unroll
-        for (int r = 0; r < R0; r++) L[lpad(jb * R0 + r)] = v[b * R0 + r];
+            for (int r = 0; r < R0; r++) L[lpad(jb * R0 + r)] = v[b * R0 + r];
+        }
     }
     __syncthreads();
     int Ns = R0;
 #pragma unroll
     for (int pass = 0; pass < S::NPASS16; pass++) {
         const bool last = (pass == S::NPASS16 - 1);
-#pragma unroll
-        for (int r = 0; r < 16; r++) v[r] = L[lpad(j + r * (N / 16))];
         const int k = j & (Ns - 1);
-        if (k != 0) {
-            float2 p[16];
-            powers16(tw[k * (N / (Ns * 16))], p);
+        if (active) {
 #pragma unroll
-            for (int r = 1; r < 16; r++) v[r] = cmul(v[r], p[r]);
+            for (int r = 0; r < 16; r++) v[r] = L[lpad(j + r * (N / 16))];
+            if (k != 0) {
+                float2 p[16];
+                powers16(tw[k * (N / (Ns * 16))], p);
+#pragma unroll
+                for (int r = 1; r < 16; r++) v[r] = cmul(v[r], p[r]);
+            }
+            dft<+1, 16>(v);
         }
-        dft<+1, 16>(v);
         if (!last) {
             __syncthreads();  // every lane has read before anyone overwrites
-            const int base = (j - k) * 16 + k;
+            if (active) {
+                const int base = (j - k) * 16 + k;
 #pragma unroll
-            for (int r = 0; r < 16; r++) L[lpad(base + r * Ns)] = v[r];
+                for (int r = 0; r < 16; r++) L[lpad(base + r * Ns)] = v[r];
+            }
             __syncthreads();
         }
         Ns *= 16;

@@ -54,7 +54,9 @@ namespace {
 constexpr int kPgaMaxN = 4096;
 constexpr int kThreads = 256;
 constexpr int kRowChunks = 16;     // y chunks of the two-pass column reductions
-constexpr int kIntThreads = 1024;  // window / integrate block; kPgaMaxN / 1024 = 4 per thread
+constexpr int kIntThreads = 1024;  // window block
+constexpr int kSpecThreads = 512;  // integrate block: kPgaMaxN / 512 = 8 values per thread,
+                                   // and the M / 16 <= 512 lanes of the spectrum transforms
 constexpr int kMinL = 5, kMaxL = 13;
 
 // Device state of one image's PGA call.
@@ -252,25 +254,47 @@ __device__ void col_partial_body(const PgaImage &p, int mode, int bx, int chunk)
     }
 }
 
+// Workgroup reductions of NT threads (NT / 64 waves): within each wave by
+// lane butterflies (every lane ends with the same value, since a + b = b + a
+// and max/min are exact), then over the waves' values through `slots`
+// (>= NT / 64 entries, shared).
+template <int NT, class T, class Op>
+__device__ __forceinline__ T block_reduce(T v, T *slots, Op op) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o));
+    __syncthreads();  // earlier readers of slots are done
+    if (lane == 0) slots[w] = v;
+    __syncthreads();
+    T t = slots[0];
+#pragma unroll
+    for (int i = 1; i < NT / 64; i++) t = op(t, slots[i]);
+    return t;
+}
+struct OpSum {
+    __device__ double operator()(double a, double b) const { return a + b; }
+};
+struct OpMax {
+    __device__ double operator()(double a, double b) const { return fmax(a, b); }
+    __device__ int operator()(int a, int b) const { return max(a, b); }
+};
+struct OpMin {
+    __device__ int operator()(int a, int b) const { return min(a, b); }
+};
+
 // One workgroup of kIntThreads: window limits from the column energy
 // (mask[j] = 1 on [left, right], 0-based): left = 1 + the last column before
 // the centre below the cut-off, right = the first column from the centre on
 // below it, minus 1 (parallel max/min reductions).
 __device__ void window_body(const PgaImage &p, double db_down) {
     if (stopped(p)) return;
-    __shared__ double rmax[kIntThreads];
-    __shared__ int rl[kIntThreads], rr[kIntThreads];
+    __shared__ double smax[kIntThreads / 64];
+    __shared__ int sl[kIntThreads / 64], sr[kIntThreads / 64];
     const int Nx = p.Nx, tid = threadIdx.x;
     const double *w = p.w;
     double m = 0.0;
     for (int x = tid; x < Nx; x += kIntThreads) m = fmax(m, w[x]);
-    rmax[tid] = m;
-    __syncthreads();
-    for (int o = kIntThreads / 2; o > 0; o >>= 1) {
-        if (tid < o) rmax[tid] = fmax(rmax[tid], rmax[tid + o]);
-        __syncthreads();
-    }
-    const double wmax = rmax[0];
+    const double wmax = block_reduce<kIntThreads>(m, smax, OpMax{});
     // dB relative to the maximum; max(dB) = 0, so the cut-off is -|db_down|
     const double cut = -fabs(db_down);
     const int center = (Nx + 1) / 2;  // 1-based ceil(Nx/2)
@@ -281,18 +305,10 @@ __device__ void window_body(const PgaImage &p, double db_down) {
             else firstb = min(firstb, j);
         }
     }
-    rl[tid] = lastb;
-    rr[tid] = firstb;
-    __syncthreads();
-    for (int o = kIntThreads / 2; o > 0; o >>= 1) {
-        if (tid < o) {
-            rl[tid] = max(rl[tid], rl[tid + o]);
-            rr[tid] = min(rr[tid], rr[tid + o]);
-        }
-        __syncthreads();
-    }
-    const int left = rl[0] + 1;                       // 0 when none is below
-    const int right = rr[0] < Nx ? rr[0] - 1 : Nx - 1;
+    lastb = block_reduce<kIntThreads>(lastb, sl, OpMax{});
+    firstb = block_reduce<kIntThreads>(firstb, sr, OpMin{});
+    const int left = lastb + 1;  // 0 when none is below
+    const int right = firstb < Nx ? firstb - 1 : Nx - 1;
     for (int j = tid; j < Nx; j += kIntThreads) p.mask[j] = (j >= left && j <= right) ? 1.f : 0.f;
 }
 
@@ -307,49 +323,43 @@ __device__ inline double unwrap_step(double d) {
     return dd;
 }
 
-__device__ inline double block_sum_1024(double v, double *red) {
-    const int tid = threadIdx.x;
-    __syncthreads();
-    red[tid] = v;
-    __syncthreads();
-    for (int o = kIntThreads / 2; o > 0; o >>= 1) {
-        if (tid < o) red[tid] += red[tid + o];
-        __syncthreads();
-    }
-    return red[0];
-}
-
-// One workgroup of kIntThreads: phi = unwrap([0 cumsum(phidot)]), detrend,
+// One workgroup of kSpecThreads: phi = unwrap([0 cumsum(phidot)]), detrend,
 // rms -> rms[it], af_ph += phi; writes e^{-j phi} (detrended) for the
 // correction.  unwrap of a cumulative sum is the cumulative sum of the
 // unwrapped steps, so phi is one block-wide prefix scan of unwrap_step.
-__device__ bool integrate_body(const PgaImage &p, int it) {
+__device__ __forceinline__ bool integrate_body(const PgaImage &p, int it) {
     if (stopped(p)) return false;
-    __shared__ double sc[kIntThreads];
-    __shared__ double red[kIntThreads];
+    constexpr int kPer = kPgaMaxN / kSpecThreads;
+    __shared__ double sc[kSpecThreads / 64];
+    __shared__ double red[kSpecThreads / 64];
     const int Nx = p.Nx, tid = threadIdx.x;
-    const int per = (Nx + kIntThreads - 1) / kIntThreads;  // <= 4
+    const int per = (Nx + kSpecThreads - 1) / kSpecThreads;  // <= kPer
     const int b = tid * per;
-    double loc[4];
+    double loc[kPer];
     double run = 0.0;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < kPer; j++) {
         const int i = b + j;
         if (j < per && i < Nx && i > 0) run += unwrap_step(p.phidot[i - 1]);
         loc[j] = run;
     }
-    sc[tid] = run;
-    __syncthreads();
-    for (int o = 1; o < kIntThreads; o <<= 1) {  // inclusive Hillis-Steele scan
-        const double v = tid >= o ? sc[tid - o] : 0.0;
-        __syncthreads();
-        sc[tid] += v;
-        __syncthreads();
+    // exclusive prefix sum of `run` over the threads: inclusive within the
+    // wave by shuffles, shifted by one lane, plus the earlier waves' totals
+    const int lane = tid & 63, wv = tid >> 6;
+    double x = run;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double y = __shfl_up(x, o);
+        if (lane >= o) x += y;
     }
-    const double off = tid > 0 ? sc[tid - 1] : 0.0;
+    if (lane == 63) sc[wv] = x;
+    __syncthreads();
+    double off = __shfl_up(x, 1);
+    if (lane == 0) off = 0.0;
+    for (int i = 0; i < wv; i++) off += sc[i];
     double sy = 0.0, sxy = 0.0;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < kPer; j++) {
         const int i = b + j;
         loc[j] += off;
         if (j < per && i < Nx) {
@@ -357,8 +367,8 @@ __device__ bool integrate_body(const PgaImage &p, int it) {
             sxy += (double)i * loc[j];
         }
     }
-    sy = block_sum_1024(sy, red);
-    sxy = block_sum_1024(sxy, red);
+    sy = block_reduce<kSpecThreads>(sy, red, OpSum{});
+    sxy = block_reduce<kSpecThreads>(sxy, red, OpSum{});
     const double n = Nx;
     const double sx = 0.5 * n * (n - 1.0), sxx = (n - 1.0) * n * (2.0 * n - 1.0) / 6.0;
     const double den = n * sxx - sx * sx;
@@ -366,12 +376,12 @@ __device__ bool integrate_body(const PgaImage &p, int it) {
     const double icpt = (sy - slope * sx) / n;
     double r = 0.0;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < kPer; j++) {
         const int i = b + j;
         loc[j] -= slope * i + icpt;
         if (j < per && i < Nx) r += loc[j] * loc[j];
     }
-    r = block_sum_1024(r, red);
+    r = block_reduce<kSpecThreads>(r, red, OpSum{});
     const double rms = sqrt(r / n);
     if (tid == 0) p.rms[it] = rms;
     // autofocus_cpu.cpp:432: an estimate below the threshold ends the loop
@@ -381,7 +391,7 @@ __device__ bool integrate_body(const PgaImage &p, int it) {
         return false;
     }
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < kPer; j++) {
         const int i = b + j;
         if (j < per && i < Nx) {
             const double v = loc[j];
@@ -566,48 +576,94 @@ __device__ __forceinline__ void row_apply_body(const PgaImage &p, int blk, float
 //   h = IDFT_N(e^{-j phi}) = conj(DFT_N(conj(e^{-j phi}))) / N   (Bluestein),
 //   b_m = h[(m - (N-1)) mod N] for m <= 2N-2, 0 up to M,  B = FFT_M(b),
 // so that IDFT_N(DFT_N(x) .* e^{-j phi}) = (x * b)[N-1 .. 2N-2] for a row x
-// zero-padded to M >= 2N-1.  One workgroup of kThreads per image, dynamic LDS
-// of M float2 (the rotation into b goes through registers, <= M/kThreads
-// values per lane).
-__device__ void apply_spectrum_body(const PgaImage &p) {
-    if (stopped(p)) return;
-    extern __shared__ float2 buf[];
-    constexpr int kMaxPer = (1 << kMaxL) / kThreads;
-    const int N = p.Nx, L = p.L, M = 1 << L;
-    for (int k = threadIdx.x; k < M; k += blockDim.x) {
-        float2 v = make_float2(0.f, 0.f);
-        if (k < N) {
-            const double2 e = p.ephi[k];
-            v = cmul(make_float2((float)e.x, -(float)e.y), p.wtab[k]);
+// zero-padded to M >= 2N-1.  The three length-M transforms run on the
+// register-resident radix-16 passes by the first M/16 lanes of the
+// workgroup (the others only join the barriers); Lr holds padded(L) float2.
+template <int L>
+__device__ __forceinline__ void spectrum_regs(int N, const double2 *__restrict__ ephi,
+                                              const float2 *__restrict__ wtab,
+                                              const float2 *__restrict__ H,
+                                              const float2 *__restrict__ twM,
+                                              float2 *__restrict__ B, float2 *Lr) {
+    using S = fft::Shape<L>;
+    constexpr int M = S::N, T = S::T, R0 = S::R0, NB0 = S::NB0;
+    const int j = threadIdx.x;
+    const bool act = j < T;
+    float2 v[16];
+    // a_k = conj(e_k) w_k for k < N, conjugated: the forward transform is
+    // conj(ifft(conj))
+#pragma unroll
+    for (int b = 0; b < NB0; b++)
+#pragma unroll
+        for (int r = 0; r < R0; r++) {
+            const int k = j + b * T + r * (M / R0);
+            float2 a = make_float2(0.f, 0.f);
+            if (act && k < N) {
+                const double2 e = ephi[k];
+                a = cmul(make_float2((float)e.x, -(float)e.y), wtab[k]);
+            }
+            v[b * R0 + r] = conjf2(a);
         }
-        buf[k] = v;
-    }
-    __syncthreads();
-    bluestein_core(buf, L, p.twM, p.H);
-    // h_i = conj(w_i buf[i]) / (M N); b_m = h[(m - (N-1)) mod N], m <= 2N-2
+    fft::ifft_regs<L>(v, Lr, j, twM, act);  // v[r] = conj(A_k), k = j + r*T
+    float2 u[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+        u[r] = act ? cmul(conjf2(v[r]), H[j + r * T]) : make_float2(0.f, 0.f);
+    fft::to_input_order<L>(u, v);
+    fft::ifft_regs<L>(v, Lr, j, twM, act);  // v[r] = c_k = M IFFT_M(A .* H)_k
+    __syncthreads();                           // every lane is done with Lr
+    // h_i = conj(w_i c_i) / (M N), i < N, staged in Lr
     const float scale = 1.0f / ((float)M * (float)N);
-    float2 keep[kMaxPer];
+    if (act) {
 #pragma unroll
-    for (int q = 0; q < kMaxPer; q++) {
-        const int m = threadIdx.x + q * blockDim.x;
-        float2 v = make_float2(0.f, 0.f);
-        if (m < M && m <= 2 * N - 2) {
-            int i = m - (N - 1);
-            if (i < 0) i += N;
-            const float2 X = cmul(buf[i], p.wtab[i]);
-            v = make_float2(X.x * scale, -X.y * scale);
+        for (int r = 0; r < 16; r++) {
+            const int i = j + r * T;
+            if (i < N) {
+                const float2 X = cmul(v[r], wtab[i]);
+                Lr[i] = make_float2(X.x * scale, -X.y * scale);
+            }
         }
-        keep[q] = v;
     }
     __syncthreads();
+    // b_m = h[(m - (N-1)) mod N] for m <= 2N-2, conjugated for the forward
+    // transform (ifft_regs starts with a barrier before it writes Lr)
 #pragma unroll
-    for (int q = 0; q < kMaxPer; q++) {
-        const int m = threadIdx.x + q * blockDim.x;
-        if (m < M) buf[m] = keep[q];
+    for (int b = 0; b < NB0; b++)
+#pragma unroll
+        for (int r = 0; r < R0; r++) {
+            const int m = j + b * T + r * (M / R0);
+            float2 bv = make_float2(0.f, 0.f);
+            if (act && m <= 2 * N - 2) {
+                int i = m - (N - 1);
+                if (i < 0) i += N;
+                bv = Lr[i];
+            }
+            v[b * R0 + r] = conjf2(bv);
+        }
+    fft::ifft_regs<L>(v, Lr, j, twM, act);  // v[r] = conj(B_k)
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) B[j + r * T] = conjf2(v[r]);
     }
-    __syncthreads();
-    lds_fft(buf, L, p.twM, false);
-    for (int m = threadIdx.x; m < M; m += blockDim.x) p.B[m] = buf[m];
+}
+
+__device__ __forceinline__ void apply_spectrum_body(const PgaImage &p, float2 *Lr) {
+    if (stopped(p)) return;
+    const int N = p.Nx;
+    const double2 *ephi = p.ephi;
+    const float2 *wtab = p.wtab, *H = p.H, *twM = p.twM;
+    float2 *B = p.B;
+    switch (p.L) {
+        case 5: spectrum_regs<5>(N, ephi, wtab, H, twM, B, Lr); break;
+        case 6: spectrum_regs<6>(N, ephi, wtab, H, twM, B, Lr); break;
+        case 7: spectrum_regs<7>(N, ephi, wtab, H, twM, B, Lr); break;
+        case 8: spectrum_regs<8>(N, ephi, wtab, H, twM, B, Lr); break;
+        case 9: spectrum_regs<9>(N, ephi, wtab, H, twM, B, Lr); break;
+        case 10: spectrum_regs<10>(N, ephi, wtab, H, twM, B, Lr); break;
+        case 11: spectrum_regs<11>(N, ephi, wtab, H, twM, B, Lr); break;
+        case 12: spectrum_regs<12>(N, ephi, wtab, H, twM, B, Lr); break;
+        case 13: spectrum_regs<13>(N, ephi, wtab, H, twM, B, Lr); break;
+    }
 }
 
 // Fused steps (fewer launches per iteration, same arithmetic):
@@ -617,8 +673,8 @@ __device__ void apply_spectrum_body(const PgaImage &p) {
 //              (one workgroup of kIntThreads);
 //   integrate: phi_dot from the phase-difference partials, the integration
 //              and detrend, and -- if the estimate is applied -- the correction
-//              kernel's spectrum (one workgroup of kIntThreads, dynamic LDS of
-//              M float2).
+//              kernel's spectrum (one workgroup of kSpecThreads, the transforms
+//              in registers through a padded LDS row).
 __device__ void stats_body(const PgaImage &p, int b) {
     if (b < p.Ny) {
         argmax_body(p, b);
@@ -640,7 +696,7 @@ __device__ void window_fused_body(const PgaImage &p, double db_down) {
     window_body(p, db_down);
 }
 
-__device__ void integrate_fused_body(const PgaImage &p, int it) {
+__device__ __forceinline__ void integrate_fused_body(const PgaImage &p, int it, float2 *Lr) {
     if (stopped(p)) return;
     const int N = p.Nx;
     for (int x = threadIdx.x; x < N - 1; x += blockDim.x) {
@@ -654,7 +710,7 @@ __device__ void integrate_fused_body(const PgaImage &p, int it) {
     __syncthreads();
     if (!integrate_body(p, it)) return;
     __syncthreads();
-    apply_spectrum_body(p);
+    apply_spectrum_body(p, Lr);
 }
 
 // Blocks of a row step for an image of Ny rows: one workgroup per row up to
@@ -681,8 +737,9 @@ __global__ __launch_bounds__(kThreads) void k1_col_partial(PgaImage p, int mode)
 __global__ __launch_bounds__(kIntThreads) void k1_window(PgaImage p, double db_down) {
     window_fused_body(p, db_down);
 }
-__global__ __launch_bounds__(kIntThreads) void k1_integrate(PgaImage p, int it) {
-    integrate_fused_body(p, it);
+__global__ __launch_bounds__(kSpecThreads) void k1_integrate(PgaImage p, int it) {
+    __shared__ float2 Lr[fft::padded(kMaxL)];
+    integrate_fused_body(p, it, Lr);
 }
 __global__ __launch_bounds__(kThreads) void k1_rows_lds(PgaImage p, int apply) {
     if (apply) row_apply_lds_body(p, blockIdx.x);
@@ -757,8 +814,9 @@ __global__ __launch_bounds__(kThreads) void kb_col_partial(PgaBatchDev b, int mo
 __global__ __launch_bounds__(kIntThreads) void kb_window(PgaBatchDev b, double db_down) {
     window_fused_body(b.img[blockIdx.x], db_down);
 }
-__global__ __launch_bounds__(kIntThreads) void kb_integrate(PgaBatchDev b, int it) {
-    integrate_fused_body(b.img[blockIdx.x], it);
+__global__ __launch_bounds__(kSpecThreads) void kb_integrate(PgaBatchDev b, int it) {
+    __shared__ float2 Lr[fft::padded(kMaxL)];
+    integrate_fused_body(b.img[blockIdx.x], it, Lr);
 }
 // Row step of one transform length: grp[g0 .. g0 + m) are the images' ids,
 // grp[g0 + m .. g0 + 2m] their block offsets.
@@ -926,7 +984,7 @@ extern "C" int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *d, void *img, double
             hipLaunchKernelGGL(k1_window, dim3(1), dim3(kIntThreads), 0, s, p, d->db_down);
             SARF_TRY(k1_row_step(p, 0, s));
             hipLaunchKernelGGL(k1_col_partial, gcp, dim3(kThreads), 0, s, p, 1);
-            hipLaunchKernelGGL(k1_integrate, dim3(1), dim3(kIntThreads), lds, s, p, it);
+            hipLaunchKernelGGL(k1_integrate, dim3(1), dim3(kSpecThreads), 0, s, p, it);
             SARF_TRY(k1_row_step(p, 1, s));
             SARF_HIP(hipGetLastError());
         }
@@ -1051,7 +1109,7 @@ extern "C" int sarf_pga_scenes(sarf_ctx *ctx, int nscenes, const sarf_pga_scene 
             for (const Group &g : groups) SARF_TRY(kb_row_step(b, g.L, g.g0, g.m, g.blocks, 0, s));
             hipLaunchKernelGGL(kb_col_partial, dim3(total(kOffColPartial)), dim3(kThreads), 0, s,
                                b, 1);
-            hipLaunchKernelGGL(kb_integrate, dim3(n), dim3(kIntThreads), lds, s, b, it);
+            hipLaunchKernelGGL(kb_integrate, dim3(n), dim3(kSpecThreads), 0, s, b, it);
             for (const Group &g : groups) SARF_TRY(kb_row_step(b, g.L, g.g0, g.m, g.blocks, 1, s));
             SARF_HIP(hipGetLastError());
         }
