@@ -67,18 +67,26 @@ def test_rc_f64_matches_oracle(ctx, nfft):
     assert oracle.rel_l2(prof, oracle.range_compress_f64(ph, nfft)) < 1e-12
 
 
-def _rc_ref(ph, nfft):
+def _rc_ref(ph, nfft, taper=None, ramp=None):
     """FP64 range compression of ph: the oracle up to 16384 bins; above, the
     numpy expression the oracle is pinned to in test_oracle.py
-    (roll(ifft(ph, N), -((N+1)//2)), i.e. cshift((N+1)/2)), since the oracle's
-    direct DFT of a length that is not a power of two costs O(N K)."""
+    (roll(ifft(ph * taper, N), -((N+1)//2)), i.e. cshift((N+1)/2), times
+    e^{-j 4 pi f0 r0 / c}), since the oracle's direct DFT of a length that is
+    not a power of two costs O(N K)."""
     if nfft <= 16384:
-        return oracle.range_compress_f64(ph, nfft)
-    x = np.fft.ifft(np.asarray(ph, np.complex128), nfft, axis=1)
-    return np.roll(x, -((nfft + 1) // 2), axis=1)
+        return oracle.range_compress_f64(ph, nfft, taper=taper, ramp=ramp)
+    x = np.asarray(ph, np.complex128)
+    if taper is not None:
+        x = x * np.asarray(taper, np.float64)[None, :]
+    x = np.roll(np.fft.ifft(x, nfft, axis=1), -((nfft + 1) // 2), axis=1)
+    if ramp is not None:
+        f0, r0 = (np.asarray(v, np.float64) for v in ramp)
+        x = x * np.exp(-1j * 4 * np.pi / 299792458.0 * f0 * r0)[:, None]
+    return x
 
 
-@pytest.mark.parametrize("nfft", [3, 37, 424, 1000, 3000, 8191, 8192 + 57, 30011, 100003])
+@pytest.mark.parametrize("nfft", [3, 37, 424, 1000, 3000, 8191, 8192 + 57, 30011, 100003,
+                                  (1 << 20) - 3])
 def test_rc_non_power_of_two_matches_oracle(ctx, nfft):
     """Nfft that is not a power of two (N_fft = K is the reference's default,
     charlotte_sar_api.hpp:173): Bluestein's identity, register-resident up
@@ -102,7 +110,7 @@ def test_rc_non_power_of_two_matches_oracle(ctx, nfft):
     r0 = 12500.0 + 0.37 * np.arange(Np)
     pt = ctx.range_compress(to_dev(ph32), nfft, taper=to_dev(taper),
                             ramp=(to_dev(f0), to_dev(r0))).cpu().numpy()
-    ref = oracle.range_compress_f64(ph32, nfft, taper=taper, ramp=(f0, r0))
+    ref = _rc_ref(ph32, nfft, taper=taper, ramp=(f0, r0))
     assert oracle.rel_l2(pt, ref) < 2e-6
 
 
