@@ -157,8 +157,8 @@ typedef struct {
     int precision;  /* SARF_BP_MIXED | SARF_BP_F64 */
     int flags;      /* SARF_BP_ACCUMULATE */
     int tile;       /* 0 = default for the pixel spacing and line of sight; else a
-                       tile id: 1..7 (lanes along y) or 11..14 (lanes along x);
-                       batches and the gradient take 1..5 and 11..14 */
+                       tile id: 1..7 (lanes along y) or 11..16 (lanes along x);
+                       batches and the gradient take 1..5 and 11..16 */
     int pulse_split;/* 0 = automatic; n >= 1: pulses in n contiguous groups summed
                        in group order -- n even groups, each a partial image, or,
                        when the image's tiles fill two rounds of resident
@@ -413,7 +413,7 @@ int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *desc,
 /* persistent launch (work items = scene tiles, largest first, drained      */
 /* through an atomic counter).  Each scene is described as for              */
 /* sarf_backproject, with its own device buffers; images are overwritten.   */
-/* tile: 0 = chosen for the finest-spaced scene, else 1..5 or 11..14.      */
+/* tile: 0 = chosen for the finest-spaced scene, else 1..5 or 11..16.      */
 /* Each distinct work list (descriptors and tile) is uploaded once (the     */
 /* call waits for the copy) into device memory of its own, kept until       */
 /* sarf_ctx_trim or destroy; a list seen before only enqueues kernels, so   */

@@ -76,7 +76,7 @@ def test_batch_entropy_reference_trajectory_model(ctx):
     assert int(np.argmin(ent)) == int(np.argmin(ref))
 
 
-@pytest.mark.parametrize("tile", [0, 4, 11, 12, 13, 14])
+@pytest.mark.parametrize("tile", [0, 4, 11, 12, 13, 14, 15, 16])
 def test_batch_entropy_matches_oracle_per_candidate(ctx, tile):
     """Every batch tile shape, both lane orientations (0 = default choice)."""
     geom, ph, grid = _los_scene()
@@ -147,14 +147,17 @@ def _misfocused(Np=40, K=64, n=32, spacing=0.15, seed=31, f0=9.6e9):
     return geom, ph, ant, ImageGrid.square(n, spacing)
 
 
-@pytest.mark.parametrize("f0,tile", [(9.6e9, 0), (35e9, 0), (9.6e9, 4), (9.6e9, 13), (9.6e9, 11),
-                                     (35e9, 13)])
-def test_adjoint_matches_fp64_oracle_gradient(ctx, f0, tile):
+@pytest.mark.parametrize("f0,tile,spacing", [(9.6e9, 0, 0.15), (35e9, 0, 0.15), (9.6e9, 4, 0.15),
+                                             (9.6e9, 13, 0.15), (9.6e9, 11, 0.15),
+                                             (35e9, 13, 0.15), (9.6e9, 16, 0.15),
+                                             (9.6e9, 15, 0.04), (35e9, 15, 0.04)])
+def test_adjoint_matches_fp64_oracle_gradient(ctx, f0, tile, spacing):
     """GPU adjoint (mixed precision) vs the FP64 CPU adjoint of the FP64 image,
     at X band and at a Ka-band carrier whose per-tile phase span exceeds the
     sine unit's input range.  Tolerance: 2e-3 relative in the 2-norm over all
-    pulses and axes."""
-    geom, ph, ant, grid = _misfocused(f0=f0)
+    pulses and axes.  (Tile 15, 256 pixels wide, needs the finer spacing for
+    its profile window to fit in LDS; at 0.15 m it is rejected.)"""
+    geom, ph, ant, grid = _misfocused(f0=f0, spacing=spacing)
     nfft = 1024
     prof_h = oracle.range_compress_f64(ph, nfft)
     prof = to_dev(prof_h.astype(np.complex64))

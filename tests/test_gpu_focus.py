@@ -229,7 +229,7 @@ def test_bp_c0_f64_matches_oracle(ctx):
     assert oracle.rel_l2(img, ref) < 1e-10
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7, 11, 12, 13, 14])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7, 11, 12, 13, 14, 15, 16])
 def test_bp_every_tile_shape(ctx, tile):
     geom, tg, ph = small_inputs()
     grid = ImageGrid(70, 45, 0.4, -0.3, 7.0, 5.5)  # ragged vs every tile
@@ -343,7 +343,7 @@ def test_bp_is_deterministic(ctx):
     assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize("tile", [0, 1, 4, 7, 11, 13])
+@pytest.mark.parametrize("tile", [0, 1, 4, 7, 11, 13, 15])
 def test_bp_smallest_images(ctx, tile):
     """Images far smaller than a tile (2x2, 2x3, 3x2, 7x2) centred on a
     target's main lobe: the tile's pixels outside the image are neither
@@ -474,7 +474,7 @@ def test_bp_scenes_uneven_batch_matches_single_calls(ctx):
         refs.append((ctx.backproject(prof, ant, r0, f0, grid, geom.maxWr).cpu().numpy(),
                      oracle.bp_f64(oracle.range_compress_f64(ph, nfft), geom.ant, geom.r0,
                                    geom.f0, grid, geom.maxWr)))
-    for tile in (0, 1, 5, 11, 13):
+    for tile in (0, 1, 5, 11, 13, 15, 16):
         imgs = ctx.backproject_scenes(scenes, tile=tile)
         for img, (single, ref) in zip(imgs, refs):
             img = img.cpu().numpy()
@@ -483,7 +483,7 @@ def test_bp_scenes_uneven_batch_matches_single_calls(ctx):
                 assert oracle.rel_l2(img, single) < 1e-5
     # the batch tuner returns an accurate tile whose batch meets the same bound
     best, ms = ctx.autotune_scenes(scenes, reps=1)
-    assert best in (1, 2, 3, 4, 5, 11, 12, 13, 14) and ms > 0
+    assert best in (1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16) and ms > 0
     imgs = ctx.backproject_scenes(scenes, tile=best)
     for img, (single, ref) in zip(imgs, refs):
         assert oracle.rel_l2(img.cpu().numpy(), ref) <= 1e-4
@@ -521,7 +521,7 @@ def test_autotune_chooses_an_accurate_shape_and_is_used_by_default():
         ant, r0, f0 = geom_dev(geom)
         prof = tctx.range_compress(to_dev(ph), cfg.nfft)
         res, img = tctx.autotune(prof, ant, r0, f0, grid, geom.maxWr, reps=2)
-        assert res["tile"] in (1, 2, 3, 4, 5, 6, 7, 11, 12, 13, 14) and res["pulse_split"] >= 1
+        assert res["tile"] in (1, 2, 3, 4, 5, 6, 7, 11, 12, 13, 14, 15, 16) and res["pulse_split"] >= 1
         assert res["candidates"] >= 2
         assert 0 < res["ms"] <= res["default_ms"] * 1.0001
         ref = oracle.bp_f64(oracle.range_compress_f64(ph, cfg.nfft), geom.ant, geom.r0, geom.f0,

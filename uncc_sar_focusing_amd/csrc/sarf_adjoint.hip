@@ -323,6 +323,8 @@ extern "C" int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *d, const 
         case 12: launch_adj_t<64, 4, true>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
         case 13: launch_adj_t<64, 8, true>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
         case 14: launch_adj_t<32, 8, true>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
+        case 15: launch_adj_t<256, 8, true>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
+        case 16: launch_adj_t<128, 8, true>(a, grid, p, ant, r0, f0, im, ent, partial, s); break;
         default:
             return fail(SARF_ERR_UNSUPPORTED, "sarf_bp_entropy_grad: tile %d not compiled", tid);
     }

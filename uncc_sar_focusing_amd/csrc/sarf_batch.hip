@@ -181,6 +181,8 @@ extern "C" int sarf_bp_batch_entropy(sarf_ctx *ctx, const sarf_bp_desc *d, const
         case 12: launch_batch_t<64, 4, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, per_basis, partial, s); break;
         case 13: launch_batch_t<64, 8, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, per_basis, partial, s); break;
         case 14: launch_batch_t<32, 8, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, per_basis, partial, s); break;
+        case 15: launch_batch_t<256, 8, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, per_basis, partial, s); break;
+        case 16: launch_batch_t<128, 8, true>(a, grid, p, ant, r0, f0, dir, basis, coef, nbasis, ncand, per_basis, partial, s); break;
         default:
             return fail(SARF_ERR_UNSUPPORTED, "sarf_bp_batch_entropy: tile %d not compiled for batches", tid);
     }

@@ -74,7 +74,7 @@ void launch_scenes_t(int grid, const SceneDev *sc, const int2 *items, int nitems
 }
 
 // Tile ids compiled for batches (at most 8 pixels per lane).
-constexpr int kSceneTiles[] = {1, 2, 3, 4, 5, 11, 12, 13, 14};
+constexpr int kSceneTiles[] = {1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16};
 bool scene_tile_ok(int id) {
     for (int t : kSceneTiles)
         if (t == id) return true;
@@ -188,6 +188,8 @@ extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scen
         case 12: launch_scenes_t<64, 4, true>(grid, dsc, ditems, nit, counter, s); break;
         case 13: launch_scenes_t<64, 8, true>(grid, dsc, ditems, nit, counter, s); break;
         case 14: launch_scenes_t<32, 8, true>(grid, dsc, ditems, nit, counter, s); break;
+        case 15: launch_scenes_t<256, 8, true>(grid, dsc, ditems, nit, counter, s); break;
+        case 16: launch_scenes_t<128, 8, true>(grid, dsc, ditems, nit, counter, s); break;
     }
     SARF_HIP(hipGetLastError());
     return SARF_OK;
