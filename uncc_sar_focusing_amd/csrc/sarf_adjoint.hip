@@ -87,8 +87,8 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
     BpArgs a, const float2 *__restrict__ prof, const double *__restrict__ ant,
     const double *__restrict__ r0, const double *__restrict__ f0, const float2 *__restrict__ img,
     const double *__restrict__ ent, double *__restrict__ partial) {
-    __shared__ BpLds L;
-    __shared__ float uz[kBpChunk];
+    __shared__ typename BpTile<TY, PX, XL>::Lds L;
+    __shared__ float uz[BpTile<TY, PX, XL>::CH];
     // blockIdx.x = split * ntiles + tile; each split covers a pulse range
     const int ntiles = a.tiles_x * a.tiles_y;
     const int split = blockIdx.x / ntiles;
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kBpThreads) void bp_entropy_adjoint(
     const PlainPos pos{ant};
 
     for (int n0 = nbeg; n0 < nend;) {
-        T.stage(a, L, prof, pos, r0, f0, n0, min(kBpChunk, nend - n0));
+        T.stage(a, L, prof, pos, r0, f0, n0, min(BpTile<TY, PX, XL>::CH, nend - n0));
         // uh_z for the chunk (stage() keeps uh_x, uh_y, 1/(2R_c) in pA)
         if (threadIdx.x < L.nc) {
             const int n = n0 + threadIdx.x;

@@ -41,7 +41,7 @@ void bp_batch_entropy(
     const double *__restrict__ r0, const double *__restrict__ f0, const double *__restrict__ dir,
     const double *__restrict__ basis, const double *__restrict__ coef, int nbasis, int ncand,
     int per_basis, double2 *__restrict__ partial) {
-    __shared__ BpLds L;
+    __shared__ typename BpTile<TY, PX, XL>::Lds L;
     __shared__ __align__(16) float fx[BpTile<TY, PX, XL>::kFx];
     __shared__ double2 red[kBpThreads / 64];
     const int ntiles = a.tiles_x * a.tiles_y;

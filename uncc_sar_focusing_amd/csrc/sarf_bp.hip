@@ -22,7 +22,7 @@ void bp_mixed(BpArgs a, const float2 *__restrict__ prof,
                                                        const double *__restrict__ r0,
                                                        const double *__restrict__ f0,
                                                        float2 *__restrict__ img) {
-    __shared__ BpLds L;
+    __shared__ typename BpTile<TY, PX, XL>::Lds L;
     __shared__ __align__(16) float fx[BpTile<TY, PX, XL>::kFx];
     // blockIdx.x = split * ntiles + tile: every split of a tile covers a
     // contiguous pulse range and writes its own partial image -- except the

@@ -67,7 +67,7 @@ constexpr int bp_waves_per_eu() {
 constexpr int bp_waves_per_eu_rt(int tl, int px, bool xl) {
     return px > 8 ? 1 : (xl && (kBpThreads / tl) * px > 32) ? 3 : 4;
 }
-constexpr int kBpChunk = 32;  // max pulses staged per chunk
+constexpr int kBpChunk = 32;  // max pulses staged per chunk (the 256-lane tile stages 64)
 constexpr int kBpCap = 2048;  // LDS window capacity, in bins (16 B each)
 static_assert(kBpChunk <= 64 && kBpCap % kBpThreads == 0, "staging layout");
 
@@ -108,13 +108,16 @@ __device__ __forceinline__ void split_range(const BpArgs &a, int s, int &nbeg, i
     nend = min(a.Np, nbeg + per);
 }
 
-struct BpLds {
+// Staged state of one chunk of up to CH pulses.
+template <int CH>
+struct BpLdsT {
+    static_assert(CH <= 64, "one wave stages a chunk");
     float4 win[kBpCap];
-    float4 pA[kBpChunk];  // uh_x, uh_y, g0, g1
-    float4 pB[kBpChunk];  // pos_c (window-local), psi_c, kappa, W - 2
-    int4 pI[kBpChunk];    // start bin, LDS offset (bins), W, interior
-    float4 pC[kBpChunk];  // separable terms (lane axis): c2l, c3l, e1, e2
-    float4 pD[kBpChunk];  // separable terms: e3, c2p, c3p (pixel axis), -
+    float4 pA[CH];  // uh_x, uh_y, g0, g1
+    float4 pB[CH];  // pos_c (window-local), psi_c, kappa, W - 2
+    int4 pI[CH];    // start bin, LDS offset (bins), W, interior
+    float4 pC[CH];  // separable terms (lane axis): c2l, c3l, e1, e2
+    float4 pD[CH];  // separable terms: e3, c2p, c3p (pixel axis), -
     int nc;   // pulses staged
     int tot;  // window bins staged
 };
@@ -176,7 +179,12 @@ struct BpTile {
     static constexpr int TX = XL ? TL : NP;        // tile extent along x
     static constexpr int TYE = XL ? NP : TL;       // tile extent along y
     static constexpr bool kSep = XL;               // separable local range
-    static constexpr int kFx = kSep ? kBpChunk * NP : 4;  // LDS floats for F_p
+    // Pulses per chunk: 64 for the 256-lane tile, whose narrow windows and
+    // 8-column F_p table leave room for them within four workgroups' LDS per
+    // CU (fewer chunk rounds: 2048^2 x 4096 pulses 6.88 -> 6.71 ms), 32 else.
+    static constexpr int CH = (XL && TL == 256 && PX <= 8) ? 64 : kBpChunk;
+    using Lds = BpLdsT<CH>;
+    static constexpr int kFx = kSep ? CH * NP : 4;  // LDS floats for F_p
     int ix0, iy0, ly, gx;
     double pcx, pcy, hx, hy;
     float qy;  // lane coordinate, tile-centred (y if !XL, x if XL)
@@ -220,13 +228,13 @@ struct BpTile {
     // L.pB, L.pI, L.pC, L.pD, packs windows, sets L.nc; then stages the
     // windows and, for separable tiles given fx, F_p per (pulse, column).
     template <class Pos>
-    __device__ __forceinline__ void stage(const BpArgs &a, BpLds &L, const float2 *__restrict__ prof,
+    __device__ __forceinline__ void stage(const BpArgs &a, Lds &L, const float2 *__restrict__ prof,
                                           const Pos &pos, const double *__restrict__ r0,
                                           const double *__restrict__ f0, int n0, int cmax,
                                           float *fx = nullptr) {
         const int tid = threadIdx.x;
         __syncthreads();  // previous users of L are done
-        // Wave 0, one lane per pulse of the chunk (cmax <= kBpChunk <= 64).
+        // Wave 0, one lane per pulse of the chunk (cmax <= CH <= 64).
         int W = 0, start = 0, interior = 0;
         if (tid < cmax) {
             const int n = n0 + tid;
@@ -278,7 +286,7 @@ struct BpTile {
             // whose windows fit in LDS are a prefix of the chunk.
             int incl = W;
 #pragma unroll
-            for (int d = 1; d < kBpChunk; d <<= 1) {
+            for (int d = 1; d < CH; d <<= 1) {
                 const int t = __shfl_up(incl, d, 64);
                 if (tid >= d) incl += t;
             }
@@ -309,12 +317,12 @@ struct BpTile {
                 if (f < tot) {
                     int c = 0;
 #pragma unroll
-                    for (int step = kBpChunk / 2; step >= 1; step >>= 1)
+                    for (int step = CH / 2; step >= 1; step >>= 1)
                         if (c + step < nc && L.pI[c + step].y <= f) c += step;
                     const int4 I = L.pI[c];
                     const int b = f - I.y;
                     const int g = I.x + b;  // profile bin; |g| < 2 Nfft
-                    const unsigned o = (unsigned)(c * a.Nfft + g);  // < kBpChunk * Nfft
+                    const unsigned o = (unsigned)(c * a.Nfft + g);  // < CH * Nfft
                     if (g >= 0 && g < a.Nfft) v0[k] = base[o];
                     if (g + 1 >= 0 && g + 1 < a.Nfft) v1[k] = base[o + 1];
                     wb[k] = b;
@@ -348,7 +356,7 @@ struct BpTile {
     // Separable form (XL tiles): per pulse the lane forms F_l, C1, C2 and
     // folds F_l into its window-position and phase offsets; per pixel
     // dl = F_p + P C1 + P^2 C2 = dR - dR_c - F_l.
-    __device__ __forceinline__ void accumulate_chunk_sep(const BpArgs &a, const BpLds &L,
+    __device__ __forceinline__ void accumulate_chunk_sep(const BpArgs &a, const Lds &L,
                                                          const float *__restrict__ fx) {
         const float invbin = a.invbin;
         const int nc = L.nc;
@@ -414,7 +422,7 @@ struct BpTile {
     }
 
     // Accumulate the staged chunk into the lane's pixels.
-    __device__ __forceinline__ void accumulate_chunk(const BpArgs &a, const BpLds &L,
+    __device__ __forceinline__ void accumulate_chunk(const BpArgs &a, const Lds &L,
                                                      const float *__restrict__ fx) {
         if constexpr (kSep) {
             accumulate_chunk_sep(a, L, fx);
@@ -482,12 +490,12 @@ struct BpTile {
 
     // All pulses in [nbeg, nend); fx: LDS of kFx floats.
     template <class Pos>
-    __device__ __forceinline__ void run(const BpArgs &a, BpLds &L, float *fx,
+    __device__ __forceinline__ void run(const BpArgs &a, Lds &L, float *fx,
                                         const float2 *__restrict__ prof, const Pos &pos,
                                         const double *__restrict__ r0,
                                         const double *__restrict__ f0, int nbeg, int nend) {
         for (int n0 = nbeg; n0 < nend;) {
-            stage(a, L, prof, pos, r0, f0, n0, min(kBpChunk, nend - n0), fx);
+            stage(a, L, prof, pos, r0, f0, n0, min(CH, nend - n0), fx);
             accumulate_chunk(a, L, fx);
             n0 += L.nc;
         }

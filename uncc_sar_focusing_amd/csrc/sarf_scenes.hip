@@ -42,7 +42,7 @@ void bp_scenes(const SceneDev *__restrict__ scenes,
                                                         const int2 *__restrict__ items,
                                                         int nitems,
                                                         unsigned int *__restrict__ counter) {
-    __shared__ BpLds L;
+    __shared__ typename BpTile<TY, PX, XL>::Lds L;
     __shared__ __align__(16) float fx[BpTile<TY, PX, XL>::kFx];
     __shared__ int sItem;
     for (;;) {
