@@ -157,17 +157,26 @@ std::vector<int> bp_tile_ids() {
     return ids;
 }
 
-// Default tile: the first tile of a measured speed ranking (MI355X, 2048^2
-// at 0.05 m and 256^2 at 0.1 m) whose line-of-sight-projected half extent
-// |los_x| hx + |los_y| hy -- the largest |dR - dR_c| over the tile, which
-// sets the FP32 rounding of the local range -- stays within 2.6 m (about
-// 3e-5 relative L2 measured).  With no LOS hint each axis counts fully and
-// the bound is 3.2 m.  The tile's worst-case window must fit in LDS.  With a
-// hint, tiles whose lanes run along the cross-range axis rank first (their
-// window gathers are LDS broadcasts): lanes along x when |los_y| >= |los_x|.
+// Default tile: among the three fastest lanes-along-x tiles (with a hint
+// along y), the cheapest by their measured relative speed divided by the
+// fraction of the tile grid's pixels that fall inside the image (a
+// 290-pixel-wide image in 256-wide tiles computes 512 columns); otherwise
+// the first of a measured speed ranking (MI355X, 2048^2 at 0.05 m and 256^2
+// at 0.1 m) -- always among the tiles whose line-of-sight-projected
+// half extent |los_x| hx + |los_y| hy -- the largest |dR - dR_c| over the
+// tile, which sets the FP32 rounding of the local range -- stays within
+// 2.6 m (about 3e-5 relative L2 measured) and whose worst-case window fits
+// in LDS.  With no LOS hint each axis counts fully and the bound is 3.2 m.
+// With a hint, tiles whose lanes run along the cross-range axis rank first
+// (their window gathers are LDS broadcasts): lanes along x when
+// |los_y| >= |los_x|.  Relative costs at 2048^2, 4096 pulses: tile 15 1.00,
+// 16 1.013, 13 1.05.  (Over 64 scenes of 150-400 pixels a side, one by one,
+// tile 13 against 15: 15 is faster on widths just under a multiple of 256,
+// slower by up to 20 % on those just above one.)
 int bp_pick_tile(const sarf_bp_desc *d, double dx, double dy) {
     static const int kRankY[] = {4, 5, 1, 3, 7, 2, 6};
     static const int kRankX[] = {15, 16, 13, 4, 5, 1, 3, 14, 12, 11, 7, 2, 6};
+    static const double kCostX[] = {1.0, 1.013, 1.05};
     const double invbin = d->Nfft / d->maxWr;
     const bool hint = (d->los_x != 0.0 || d->los_y != 0.0);
     const double lx = hint ? std::fabs(d->los_x) : 1.0, ly = hint ? std::fabs(d->los_y) : 1.0;
@@ -175,6 +184,8 @@ int bp_pick_tile(const sarf_bp_desc *d, double dx, double dy) {
     const bool xlanes = hint && ly >= lx;
     const int *rank = xlanes ? kRankX : kRankY;
     const int nrank = xlanes ? (int)(sizeof(kRankX) / sizeof(int)) : (int)(sizeof(kRankY) / sizeof(int));
+    int best = 0;
+    double best_cost = 0.0;
     for (int k = 0; k < nrank; k++) {
         const int id = rank[k];
         for (const TileShape &t : kTiles) {
@@ -182,10 +193,20 @@ int bp_pick_tile(const sarf_bp_desc *d, double dx, double dy) {
             const double hx = 0.5 * (t.tx - 1) * dx, hy = 0.5 * (t.ty - 1) * dy;
             if (lx * hx + ly * hy > bound) continue;
             if (2.0 * (hx + hy) * invbin + 8 > kBpCap) continue;
-            return id;
+            if (!xlanes || k >= 3) {
+                if (best == 0) return id;  // below the measured three: rank order
+                continue;
+            }
+            const double ux = (double)d->Nx / (((d->Nx + t.tx - 1) / t.tx) * (double)t.tx);
+            const double uy = (double)d->Ny / (((d->Ny + t.ty - 1) / t.ty) * (double)t.ty);
+            const double cost = kCostX[k] / (ux * uy);
+            if (best == 0 || cost < best_cost) {
+                best = id;
+                best_cost = cost;
+            }
         }
     }
-    return 1;
+    return best > 0 ? best : 1;
 }
 
 int bp_tile_shape(int id, int *tx, int *ty);
