@@ -170,7 +170,7 @@ std::vector<int> bp_tile_ids() {
 // With a hint, tiles whose lanes run along the cross-range axis rank first
 // (their window gathers are LDS broadcasts): lanes along x when
 // |los_y| >= |los_x|.  Relative costs at 2048^2, 4096 pulses: tile 15 1.00,
-// 16 1.013, 13 1.05.  (Over 64 scenes of 150-400 pixels a side, one by one,
+// 16 1.013, 13 1.05.  (Over 64 scenes of 128-1024 pixels a side, one by one,
 // tile 13 against 15: 15 is faster on widths just under a multiple of 256,
 // slower by up to 20 % on those just above one.)
 int bp_pick_tile(const sarf_bp_desc *d, double dx, double dy) {
