@@ -20,6 +20,8 @@ constexpr double kClight = 299792458.0;   // uncc_sar_focusing.hpp:34
 constexpr double kPi = 3.141592653589793; // uncc_sar_focusing.hpp:36
 // Candidates per level of sarf_grid_search.
 constexpr long kGridMaxCandidates = 1L << 21;
+// Key offset of the PGA Bluestein tables in sarf_ctx::rc_blue.
+constexpr long kPgaTableKey = 1L << 40;
 
 // Records a printf-style message as the calling thread's last error and
 // returns `code`, so call sites read `return fail(SARF_ERR_INVALID, ...)`.
@@ -59,8 +61,9 @@ struct sarf_ctx {
     // DFT twiddle tables e^{+j 2 pi i / N}, keyed by N
     std::map<long, float2 *> tw32;
     std::map<long, double2 *> tw64;
-    // Bluestein range-compression tables, keyed by the transform length N
-    // (FP32 tables) or -N (FP64 tables)
+    // Bluestein tables, one allocation per key: range compression of
+    // length N in FP32 (key N) or FP64 (key -N); PGA rows of length N
+    // (key sarf::kPgaTableKey + N)
     std::map<long, void *> rc_blue;
     // named scratch buffers (partial images, reductions, staging); a buffer
     // that has to grow is retired, not freed, so work enqueued earlier --
@@ -120,6 +123,12 @@ int twiddles_any_f32(sarf_ctx *ctx, long n, const float2 **out);
 int twiddles_any_f64(sarf_ctx *ctx, long n, const double2 **out);
 
 int ilog2_exact(long n);  // log2 if n is a power of two, else -1
+
+// Bluestein chirp and kernel spectrum of a length-N DFT with transform
+// length M = 2^L >= 2N - 1, in FP64: c_k = e^{+j pi k^2 / N} (k < N) and
+// C = FFT_M(h), h_d = conj(c_d) for d < N, h_{M-d} = conj(c_d) for 0 < d < N.
+void bluestein_f64(int N, int L, std::vector<double> &cr, std::vector<double> &ci,
+                   std::vector<double> &hr, std::vector<double> &hi);
 
 // Internal launchers shared between translation units.
 int launch_range_compress(sarf_ctx *ctx, const sarf_rc_desc *d, const void *ph,

@@ -26,8 +26,9 @@
 // self-sorting radix-16 passes through a padded LDS row (4 and 2 rows per
 // workgroup at M = 1024, 2048); one transform's output feeds the next in
 // registers.  Shorter rows use one 256-thread workgroup per row with the row
-// in LDS (radix-2, in place), which has the lower latency there.  FFT(chirp)
-// is computed once per call (radix-2 in LDS).  The shift, mask and
+// in LDS (radix-2, in place), which has the lower latency there.  The chirp
+// and FFT(chirp) are tables per row length, evaluated once in FP64 on the
+// host and kept in the context.  The shift, mask and
 // ifftshift of step 3 fold into the load (index remap) and one phase factor
 // per (k, y); step 6 is a forward transform, the e^{-j phi} multiply and an
 // inverse transform in the same workgroup, in place.  Reductions over rows
@@ -65,13 +66,11 @@ struct PgaImage {
     double *af_ph, *rms;   // outputs: Nx, iters
     int *shift;            // Ny
     double *pa, *pb;       // kRowChunks x Nx partial column sums
-    double *w, *phidot;    // Nx
-    double2 *ephi;         // Nx
     float *mask;           // Nx
-    float2 *H;             // M: FFT of the conjugate chirp
     float2 *B;             // M: spectrum of this iteration's correction kernel
-    float2 *wtab;          // N: chirp w_n = e^{-j pi n^2 / N}
-    const float2 *twM, *twN, *ch2n;  // e^{+j 2 pi t/M}, /N, /(2N)
+    const float2 *wtab;    // N: chirp w_n = e^{-j pi n^2 / N} (context table)
+    const float2 *H;       // M: FFT of the conjugate chirp (context table)
+    const float2 *twM, *twN;  // e^{+j 2 pi t/M}, /N
     int *stop;             // 1 once an iteration's rms fell below rms_stop
     double rms_stop;       // > 0: early stop threshold; 0: fixed iteration count
     int Nx, Ny, L, iters;
@@ -117,13 +116,6 @@ __device__ void lds_fft(float2 *buf, int L, const float2 *__restrict__ tw, bool 
     }
 }
 
-// Bluestein chirp w_m = e^{-j pi m^2 / N} from ch2n[t] = e^{+j 2 pi t/(2N)}.
-__device__ __forceinline__ float2 chirp(int m, int N, const float2 *__restrict__ ch2n) {
-    const long t = ((long)m * m) % (2L * N);
-    const float2 c = ch2n[t];
-    return make_float2(c.x, -c.y);
-}
-
 __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
 
 // buf[0..N) holds a_n = x_n w_n, buf[N..M) zeros; on return buf[k] (k < N)
@@ -139,27 +131,13 @@ __device__ __forceinline__ void bluestein_core(float2 *buf, int L, const float2 
 
 // ---------------------------------------------------------------- bodies
 
-// af_ph = 0 and, with `spectrum`, H = FFT_M(h), h_m = conj(w_m) for m < N,
-// h_{M-m} = conj(w_m) for 0 < m < N (one workgroup of kThreads, dynamic LDS
-// of at least M float2).
-__device__ void setup_body(const PgaImage &p, bool spectrum) {
-    extern __shared__ float2 buf[];
-    for (int i = threadIdx.x; i < p.Nx; i += kThreads) p.af_ph[i] = 0.0;
+// af_ph = 0, the stop flag cleared and, with an rms threshold, rms = NaN
+// (iterations that do not run report NaN); any workgroup size.
+__device__ void init_body(const PgaImage &p) {
+    for (int i = threadIdx.x; i < p.Nx; i += blockDim.x) p.af_ph[i] = 0.0;
     if (threadIdx.x == 0) *p.stop = 0;
-    if (p.rms_stop > 0.0)  // iterations that do not run report NaN
-        for (int i = threadIdx.x; i < p.iters; i += kThreads) p.rms[i] = __builtin_nan("");
-    if (!spectrum) return;
-    const int N = p.Nx, L = p.L, M = 1 << L;
-    for (int i = threadIdx.x; i < N; i += kThreads) p.wtab[i] = chirp(i, N, p.ch2n);
-    for (int i = threadIdx.x; i < M; i += kThreads) {
-        float2 v = make_float2(0.f, 0.f);
-        if (i < N) v = chirp(i, N, p.ch2n);
-        else if (i > M - N) v = chirp(M - i, N, p.ch2n);
-        buf[i] = conjf2(v);
-    }
-    __syncthreads();
-    lds_fft(buf, L, p.twM, false);
-    for (int i = threadIdx.x; i < M; i += kThreads) p.H[i] = buf[i];
+    if (p.rms_stop > 0.0)
+        for (int i = threadIdx.x; i < p.iters; i += blockDim.x) p.rms[i] = __builtin_nan("");
 }
 
 // dst[c*R + r] = src[r*C + c] for an R x C complex matrix; 64 x 64 tile
@@ -282,16 +260,14 @@ struct OpMin {
     __device__ int operator()(int a, int b) const { return min(a, b); }
 };
 
-// One workgroup of kIntThreads: window limits from the column energy
-// (mask[j] = 1 on [left, right], 0-based): left = 1 + the last column before
-// the centre below the cut-off, right = the first column from the centre on
-// below it, minus 1 (parallel max/min reductions).
-__device__ void window_body(const PgaImage &p, double db_down) {
-    if (stopped(p)) return;
+// One workgroup of kIntThreads: window limits from the column energy w
+// (Nx, in LDS; mask[j] = 1 on [left, right], 0-based): left = 1 + the last
+// column before the centre below the cut-off, right = the first column from
+// the centre on below it, minus 1 (parallel max/min reductions).
+__device__ void window_body(const PgaImage &p, const double *w, double db_down) {
     __shared__ double smax[kIntThreads / 64];
     __shared__ int sl[kIntThreads / 64], sr[kIntThreads / 64];
     const int Nx = p.Nx, tid = threadIdx.x;
-    const double *w = p.w;
     double m = 0.0;
     for (int x = tid; x < Nx; x += kIntThreads) m = fmax(m, w[x]);
     const double wmax = block_reduce<kIntThreads>(m, smax, OpMax{});
@@ -323,13 +299,17 @@ __device__ inline double unwrap_step(double d) {
     return dd;
 }
 
-// One workgroup of kSpecThreads: phi = unwrap([0 cumsum(phidot)]), detrend,
-// rms -> rms[it], af_ph += phi; writes e^{-j phi} (detrended) for the
-// correction.  unwrap of a cumulative sum is the cumulative sum of the
-// unwrapped steps, so phi is one block-wide prefix scan of unwrap_step.
-__device__ __forceinline__ bool integrate_body(const PgaImage &p, int it) {
-    if (stopped(p)) return false;
-    constexpr int kPer = kPgaMaxN / kSpecThreads;
+constexpr int kPer = kPgaMaxN / kSpecThreads;  // x values per thread of the integrate block
+
+// One workgroup of kSpecThreads: phi = unwrap([0 cumsum(phidot)]) from phid
+// (Nx - 1 values, LDS), detrend, rms -> rms[it], af_ph += phi (af0: this
+// thread's af_ph entries x = tid * per + j, loaded ahead); writes
+// e^{-j phi} of the detrended phi for the correction to eph (Nx, LDS) as
+// (cos phi, sin phi) in FP32.  unwrap of a cumulative sum is the cumulative
+// sum of the unwrapped steps, so phi is one block-wide prefix scan of
+// unwrap_step.
+__device__ __forceinline__ bool integrate_body(const PgaImage &p, int it, const double *phid,
+                                               const double (&af0)[kPer], float2 *eph) {
     __shared__ double sc[kSpecThreads / 64];
     __shared__ double red[kSpecThreads / 64];
     const int Nx = p.Nx, tid = threadIdx.x;
@@ -340,7 +320,7 @@ __device__ __forceinline__ bool integrate_body(const PgaImage &p, int it) {
 #pragma unroll
     for (int j = 0; j < kPer; j++) {
         const int i = b + j;
-        if (j < per && i < Nx && i > 0) run += unwrap_step(p.phidot[i - 1]);
+        if (j < per && i < Nx && i > 0) run += unwrap_step(phid[i - 1]);
         loc[j] = run;
     }
     // exclusive prefix sum of `run` over the threads: inclusive within the
@@ -395,10 +375,10 @@ __device__ __forceinline__ bool integrate_body(const PgaImage &p, int it) {
         const int i = b + j;
         if (j < per && i < Nx) {
             const double v = loc[j];
-            p.af_ph[i] += v;
+            p.af_ph[i] = af0[j] + v;
             double sn, cs;
             sincos(v, &sn, &cs);
-            p.ephi[i] = make_double2(cs, -sn);
+            eph[i] = make_float2((float)cs, (float)sn);
         }
     }
     return true;
@@ -579,11 +559,42 @@ __device__ __forceinline__ void row_apply_body(const PgaImage &p, int blk, float
 // zero-padded to M >= 2N-1.  The three length-M transforms run on the
 // register-resident radix-16 passes by the first M/16 lanes of the
 // workgroup (the others only join the barriers); Lr holds padded(L) float2.
+
+// The tables the spectrum reads, loaded by lane j < T = M/16 ahead of the
+// integration: win[b*R0 + r] = w_k, k = j + b*T + r*(M/R0) (the transform
+// input order), wout[r] = w_i and h[r] = H_i, i = j + r*T (its output
+// order), zero for k, i >= N; and the lane's pass twiddles.
 template <int L>
-__device__ __forceinline__ void spectrum_regs(int N, const double2 *__restrict__ ephi,
-                                              const float2 *__restrict__ wtab,
-                                              const float2 *__restrict__ H,
-                                              const float2 *__restrict__ twM,
+struct SpecTabs {
+    float2 win[16], wout[16], h[16];
+    fft::PassTw<L> tw;
+};
+template <int L>
+__device__ __forceinline__ void load_spec_tabs(const PgaImage &p, SpecTabs<L> &t) {
+    using S = fft::Shape<L>;
+    constexpr int M = S::N, T = S::T, R0 = S::R0, NB0 = S::NB0;
+    const int j = threadIdx.x, N = p.Nx;
+    const bool act = j < T;
+    const float2 z = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int b = 0; b < NB0; b++)
+#pragma unroll
+        for (int r = 0; r < R0; r++) {
+            const int k = j + b * T + r * (M / R0);
+            t.win[b * R0 + r] = (act && k < N) ? p.wtab[k] : z;
+        }
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int i = j + r * T;
+        t.wout[r] = (act && i < N) ? p.wtab[i] : z;
+        t.h[r] = act ? p.H[i] : z;
+    }
+    fft::load_pass_tw<L>(p.twM, j, t.tw, act);
+}
+
+// eph: e^{-j phi} as (cos phi, sin phi), N values in LDS.
+template <int L>
+__device__ __forceinline__ void spectrum_regs(int N, const float2 *eph, const SpecTabs<L> &t,
                                               float2 *__restrict__ B, float2 *Lr) {
     using S = fft::Shape<L>;
     constexpr int M = S::N, T = S::T, R0 = S::R0, NB0 = S::NB0;
@@ -598,19 +609,15 @@ __device__ __forceinline__ void spectrum_regs(int N, const double2 *__restrict__
         for (int r = 0; r < R0; r++) {
             const int k = j + b * T + r * (M / R0);
             float2 a = make_float2(0.f, 0.f);
-            if (act && k < N) {
-                const double2 e = ephi[k];
-                a = cmul(make_float2((float)e.x, -(float)e.y), wtab[k]);
-            }
+            if (act && k < N) a = cmul(eph[k], t.win[b * R0 + r]);
             v[b * R0 + r] = conjf2(a);
         }
-    fft::ifft_regs<L>(v, Lr, j, twM, act);  // v[r] = conj(A_k), k = j + r*T
+    fft::ifft_regs<L>(v, Lr, j, t.tw, act);  // v[r] = conj(A_k), k = j + r*T
     float2 u[16];
 #pragma unroll
-    for (int r = 0; r < 16; r++)
-        u[r] = act ? cmul(conjf2(v[r]), H[j + r * T]) : make_float2(0.f, 0.f);
+    for (int r = 0; r < 16; r++) u[r] = act ? cmul(conjf2(v[r]), t.h[r]) : make_float2(0.f, 0.f);
     fft::to_input_order<L>(u, v);
-    fft::ifft_regs<L>(v, Lr, j, twM, act);  // v[r] = c_k = M IFFT_M(A .* H)_k
+    fft::ifft_regs<L>(v, Lr, j, t.tw, act);  // v[r] = c_k = M IFFT_M(A .* H)_k
     __syncthreads();                           // every lane is done with Lr
     // h_i = conj(w_i c_i) / (M N), i < N, staged in Lr
     const float scale = 1.0f / ((float)M * (float)N);
@@ -619,7 +626,7 @@ __device__ __forceinline__ void spectrum_regs(int N, const double2 *__restrict__
         for (int r = 0; r < 16; r++) {
             const int i = j + r * T;
             if (i < N) {
-                const float2 X = cmul(v[r], wtab[i]);
+                const float2 X = cmul(v[r], t.wout[r]);
                 Lr[i] = make_float2(X.x * scale, -X.y * scale);
             }
         }
@@ -640,29 +647,10 @@ __device__ __forceinline__ void spectrum_regs(int N, const double2 *__restrict__
             }
             v[b * R0 + r] = conjf2(bv);
         }
-    fft::ifft_regs<L>(v, Lr, j, twM, act);  // v[r] = conj(B_k)
+    fft::ifft_regs<L>(v, Lr, j, t.tw, act);  // v[r] = conj(B_k)
     if (act) {
 #pragma unroll
         for (int r = 0; r < 16; r++) B[j + r * T] = conjf2(v[r]);
-    }
-}
-
-__device__ __forceinline__ void apply_spectrum_body(const PgaImage &p, float2 *Lr) {
-    if (stopped(p)) return;
-    const int N = p.Nx;
-    const double2 *ephi = p.ephi;
-    const float2 *wtab = p.wtab, *H = p.H, *twM = p.twM;
-    float2 *B = p.B;
-    switch (p.L) {
-        case 5: spectrum_regs<5>(N, ephi, wtab, H, twM, B, Lr); break;
-        case 6: spectrum_regs<6>(N, ephi, wtab, H, twM, B, Lr); break;
-        case 7: spectrum_regs<7>(N, ephi, wtab, H, twM, B, Lr); break;
-        case 8: spectrum_regs<8>(N, ephi, wtab, H, twM, B, Lr); break;
-        case 9: spectrum_regs<9>(N, ephi, wtab, H, twM, B, Lr); break;
-        case 10: spectrum_regs<10>(N, ephi, wtab, H, twM, B, Lr); break;
-        case 11: spectrum_regs<11>(N, ephi, wtab, H, twM, B, Lr); break;
-        case 12: spectrum_regs<12>(N, ephi, wtab, H, twM, B, Lr); break;
-        case 13: spectrum_regs<13>(N, ephi, wtab, H, twM, B, Lr); break;
     }
 }
 
@@ -686,31 +674,58 @@ __device__ void stats_body(const PgaImage &p, int b) {
 
 __device__ void window_fused_body(const PgaImage &p, double db_down) {
     if (stopped(p)) return;
+    __shared__ double w[kPgaMaxN];
     const int N = p.Nx;
     for (int x = threadIdx.x; x < N; x += blockDim.x) {
         double a = 0.0;
         for (int c = 0; c < kRowChunks; c++) a += p.pa[(size_t)c * N + x];
-        p.w[x] = a;
+        w[x] = a;
     }
     __syncthreads();
-    window_body(p, db_down);
+    window_body(p, w, db_down);
 }
 
-__device__ __forceinline__ void integrate_fused_body(const PgaImage &p, int it, float2 *Lr) {
-    if (stopped(p)) return;
-    const int N = p.Nx;
-    for (int x = threadIdx.x; x < N - 1; x += blockDim.x) {
+// The integrate step for transform length M = 2^L; every global read it
+// needs but the partial sums (the spectrum's tables, af_ph) is issued first.
+// Lr: padded(L) float2; phid: kPgaMaxN doubles; eph: kPgaMaxN float2 (LDS).
+template <int L>
+__device__ __forceinline__ void integrate_l(const PgaImage &p, int it, float2 *Lr, double *phid,
+                                            float2 *eph) {
+    const int N = p.Nx, tid = threadIdx.x;
+    SpecTabs<L> t;
+    load_spec_tabs<L>(p, t);
+    const int per = (N + kSpecThreads - 1) / kSpecThreads, x0 = tid * per;
+    double af0[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) af0[j] = (j < per && x0 + j < N) ? p.af_ph[x0 + j] : 0.0;
+    for (int x = tid; x < N - 1; x += blockDim.x) {
         double a = 0.0, b = 0.0;
         for (int c = 0; c < kRowChunks; c++) {
             a += p.pa[(size_t)c * N + x];
             b += p.pb[(size_t)c * N + x];
         }
-        p.phidot[x] = b > 0.0 ? a / b : 0.0;
+        phid[x] = b > 0.0 ? a / b : 0.0;
     }
     __syncthreads();
-    if (!integrate_body(p, it)) return;
+    if (!integrate_body(p, it, phid, af0, eph)) return;
     __syncthreads();
-    apply_spectrum_body(p, Lr);
+    spectrum_regs<L>(N, eph, t, p.B, Lr);
+}
+
+__device__ __forceinline__ void integrate_fused_body(const PgaImage &p, int it, float2 *Lr,
+                                                     double *phid, float2 *eph) {
+    if (stopped(p)) return;
+    switch (p.L) {
+        case 5: integrate_l<5>(p, it, Lr, phid, eph); break;
+        case 6: integrate_l<6>(p, it, Lr, phid, eph); break;
+        case 7: integrate_l<7>(p, it, Lr, phid, eph); break;
+        case 8: integrate_l<8>(p, it, Lr, phid, eph); break;
+        case 9: integrate_l<9>(p, it, Lr, phid, eph); break;
+        case 10: integrate_l<10>(p, it, Lr, phid, eph); break;
+        case 11: integrate_l<11>(p, it, Lr, phid, eph); break;
+        case 12: integrate_l<12>(p, it, Lr, phid, eph); break;
+        case 13: integrate_l<13>(p, it, Lr, phid, eph); break;
+    }
 }
 
 // Blocks of a row step for an image of Ny rows: one workgroup per row up to
@@ -723,10 +738,10 @@ int row_blocks(int L, int Ny) {
 
 // ------------------------------------------------- kernels: one image
 
-__global__ __launch_bounds__(kThreads) void k1_setup(PgaImage p, int spectrum) {
-    setup_body(p, spectrum != 0);
-}
+__global__ __launch_bounds__(kThreads) void k1_setup(PgaImage p) { init_body(p); }
+// The forward transpose's first block also runs init_body.
 __global__ __launch_bounds__(kThreads) void k1_transpose(PgaImage p, int forward) {
+    if (forward && blockIdx.x == 0 && blockIdx.y == 0) init_body(p);
     if (forward) transpose_body(p.img, p.Nx, p.Ny, p.At, blockIdx.x, blockIdx.y);
     else transpose_body(p.At, p.Ny, p.Nx, p.img, blockIdx.x, blockIdx.y);
 }
@@ -739,7 +754,9 @@ __global__ __launch_bounds__(kIntThreads) void k1_window(PgaImage p, double db_d
 }
 __global__ __launch_bounds__(kSpecThreads) void k1_integrate(PgaImage p, int it) {
     __shared__ float2 Lr[fft::padded(kMaxL)];
-    integrate_fused_body(p, it, Lr);
+    __shared__ double phid[kPgaMaxN];
+    __shared__ float2 eph[kPgaMaxN];
+    integrate_fused_body(p, it, Lr, phid, eph);
 }
 __global__ __launch_bounds__(kThreads) void k1_rows_lds(PgaImage p, int apply) {
     if (apply) row_apply_lds_body(p, blockIdx.x);
@@ -753,6 +770,10 @@ __global__ __launch_bounds__(fft::group_threads(LOG2M)) void k1_rows(PgaImage p,
 }
 
 // ------------------------------------------------ kernels: many images
+
+// The kernels below copy their image's PgaImage into registers before any
+// other work: a field read after a barrier could otherwise become a vector
+// load (the barrier's fence may clobber memory for all the compiler knows).
 
 // Image i owns blocks [off[i], off[i+1]) of a launch over n images.
 __device__ __forceinline__ int owner(const int *__restrict__ off, int n, int blk) {
@@ -775,14 +796,17 @@ struct PgaBatchDev {
     int n;
 };
 
-__global__ __launch_bounds__(kThreads) void kb_setup(PgaBatchDev b, int spectrum) {
-    setup_body(b.img[blockIdx.x], spectrum != 0);
+__global__ __launch_bounds__(kThreads) void kb_setup(PgaBatchDev b) {
+    const PgaImage p = b.img[blockIdx.x];
+    init_body(p);
 }
+// The forward transpose's first block of every image also runs init_body.
 __global__ __launch_bounds__(kThreads) void kb_transpose(PgaBatchDev b, int forward) {
     const int *off = b.off + kOffTranspose * (b.n + 1);
     const int i = owner(off, b.n, blockIdx.x);
-    const PgaImage &p = b.img[i];
+    const PgaImage p = b.img[i];
     const int l = blockIdx.x - off[i];
+    if (forward && l == 0) init_body(p);
     const int R = forward ? p.Nx : p.Ny, C = forward ? p.Ny : p.Nx;
     const int gx = (C + 63) / 64;
     if (forward) transpose_body(p.img, R, C, p.At, l % gx, l / gx);
@@ -794,12 +818,13 @@ __global__ __launch_bounds__(kThreads) void kb_stats(PgaBatchDev b, int rows) {
     if ((int)blockIdx.x < rows) {
         const int *off = b.off + kOffRows * (b.n + 1);
         const int i = owner(off, b.n, blockIdx.x);
-        argmax_body(b.img[i], blockIdx.x - off[i]);
+        const PgaImage p = b.img[i];
+        argmax_body(p, blockIdx.x - off[i]);
     } else {
         const int *off = b.off + kOffColPartial * (b.n + 1);
         const int bb = blockIdx.x - rows;
         const int i = owner(off, b.n, bb);
-        const PgaImage &p = b.img[i];
+        const PgaImage p = b.img[i];
         const int l = bb - off[i], gx = (p.Nx + 63) / 64;
         col_partial_body(p, 0, l % gx, l / gx);
     }
@@ -807,23 +832,27 @@ __global__ __launch_bounds__(kThreads) void kb_stats(PgaBatchDev b, int rows) {
 __global__ __launch_bounds__(kThreads) void kb_col_partial(PgaBatchDev b, int mode) {
     const int *off = b.off + kOffColPartial * (b.n + 1);
     const int i = owner(off, b.n, blockIdx.x);
-    const PgaImage &p = b.img[i];
+    const PgaImage p = b.img[i];
     const int l = blockIdx.x - off[i], gx = (p.Nx + 63) / 64;
     col_partial_body(p, mode, l % gx, l / gx);
 }
 __global__ __launch_bounds__(kIntThreads) void kb_window(PgaBatchDev b, double db_down) {
-    window_fused_body(b.img[blockIdx.x], db_down);
+    const PgaImage p = b.img[blockIdx.x];
+    window_fused_body(p, db_down);
 }
 __global__ __launch_bounds__(kSpecThreads) void kb_integrate(PgaBatchDev b, int it) {
     __shared__ float2 Lr[fft::padded(kMaxL)];
-    integrate_fused_body(b.img[blockIdx.x], it, Lr);
+    __shared__ double phid[kPgaMaxN];
+    __shared__ float2 eph[kPgaMaxN];
+    const PgaImage p = b.img[blockIdx.x];
+    integrate_fused_body(p, it, Lr, phid, eph);
 }
 // Row step of one transform length: grp[g0 .. g0 + m) are the images' ids,
 // grp[g0 + m .. g0 + 2m] their block offsets.
 __global__ __launch_bounds__(kThreads) void kb_rows_lds(PgaBatchDev b, int g0, int m, int apply) {
     const int *ids = b.grp + g0, *off = ids + m;
     const int k = owner(off, m, blockIdx.x);
-    const PgaImage &p = b.img[ids[k]];
+    const PgaImage p = b.img[ids[k]];
     if (apply) row_apply_lds_body(p, blockIdx.x - off[k]);
     else row_dft_lds_body(p, blockIdx.x - off[k]);
 }
@@ -837,7 +866,7 @@ __attribute__((amdgpu_waves_per_eu(LOG2M <= 12 ? 4 : 1))) void kb_rows(PgaBatchD
                                                                         int m, int apply) {
     const int *ids = b.grp + g0, *off = ids + m;
     const int k = owner(off, m, blockIdx.x);
-    const PgaImage &p = b.img[ids[k]];
+    const PgaImage p = b.img[ids[k]];
     __shared__ float2 lds[row_lds_elems<LOG2M>()];
     if (apply) row_apply_body<LOG2M>(p, blockIdx.x - off[k], lds);
     else row_dft_body<LOG2M>(p, blockIdx.x - off[k], lds);
@@ -856,9 +885,7 @@ size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 // Bytes of one image's small state, laid out by carve().
 size_t small_bytes(int Nx, int Ny, int L) {
     return align256(sizeof(int) * Ny) + 2 * align256(sizeof(double) * kRowChunks * Nx) +
-           2 * align256(sizeof(double) * Nx) + align256(sizeof(double2) * Nx) +
-           align256(sizeof(float) * Nx) + align256(sizeof(float2) * ((size_t)1 << L)) +
-           align256(sizeof(float2) * Nx) + align256(sizeof(int)) +
+           align256(sizeof(float) * Nx) + align256(sizeof(int)) +
            align256(sizeof(float2) * ((size_t)1 << L));
 }
 
@@ -871,20 +898,50 @@ void carve(PgaImage &p, char *base) {
     p.shift = (int *)take(sizeof(int) * p.Ny);
     p.pa = (double *)take(sizeof(double) * kRowChunks * p.Nx);
     p.pb = (double *)take(sizeof(double) * kRowChunks * p.Nx);
-    p.w = (double *)take(sizeof(double) * p.Nx);
-    p.phidot = (double *)take(sizeof(double) * p.Nx);
-    p.ephi = (double2 *)take(sizeof(double2) * p.Nx);
     p.mask = (float *)take(sizeof(float) * p.Nx);
-    p.H = (float2 *)take(sizeof(float2) * ((size_t)1 << p.L));
-    p.wtab = (float2 *)take(sizeof(float2) * p.Nx);
     p.stop = (int *)take(sizeof(int));
     p.B = (float2 *)take(sizeof(float2) * ((size_t)1 << p.L));
+}
+
+// Row tables of length N (M = 2^L), cached in the context: [w (N) | H (M)],
+// w_n = e^{-j pi n^2 / N} and H = FFT_M(g), g_m = conj(w_m) for m < N,
+// g_{M-m} = conj(w_m) for 0 < m < N.  With c and C of bluestein_f64,
+// w = conj(c) and g = conj(h), so H_k = conj(C_{(M-k) mod M}); FP64, rounded
+// once.
+int row_tables(sarf_ctx *ctx, int N, int L, const float2 **w, const float2 **H) {
+    const int M = 1 << L;
+    const long key = kPgaTableKey + N;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->rc_blue.find(key);
+    if (it == ctx->rc_blue.end()) {
+        std::vector<double> cr, ci, hr, hi;
+        bluestein_f64(N, L, cr, ci, hr, hi);
+        std::vector<float2> t((size_t)N + M);
+        for (int n = 0; n < N; n++) t[n] = make_float2((float)cr[n], (float)-ci[n]);
+        for (int k = 0; k < M; k++) {
+            const int q = (M - k) & (M - 1);
+            t[(size_t)N + k] = make_float2((float)hr[q], (float)-hi[q]);
+        }
+        void *dv = nullptr;
+        hipError_t e = hipMalloc(&dv, sizeof(float2) * t.size());
+        if (e != hipSuccess)
+            return fail(SARF_ERR_NOMEM, "pga: row tables: %s", hipGetErrorString(e));
+        e = hipMemcpy(dv, t.data(), sizeof(float2) * t.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(dv);
+            return fail(SARF_ERR_HIP, "pga: row tables: %s", hipGetErrorString(e));
+        }
+        it = ctx->rc_blue.emplace(key, dv).first;
+    }
+    *w = (const float2 *)it->second;
+    *H = (const float2 *)it->second + N;
+    return SARF_OK;
 }
 
 int tables(sarf_ctx *ctx, PgaImage &p) {
     SARF_TRY(twiddles_any_f32(ctx, 1L << p.L, &p.twM));
     SARF_TRY(twiddles_any_f32(ctx, p.Nx, &p.twN));
-    SARF_TRY(twiddles_any_f32(ctx, 2L * p.Nx, &p.ch2n));
+    SARF_TRY(row_tables(ctx, p.Nx, p.L, &p.wtab, &p.H));
     return SARF_OK;
 }
 
@@ -972,8 +1029,7 @@ extern "C" int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *d, void *img, double
     p.G = (float2 *)bufG;
     carve(p, (char *)bufS);
     SARF_TRY(tables(ctx, p));
-    const size_t lds = sizeof(float2) * ((size_t)1 << p.L);
-    hipLaunchKernelGGL(k1_setup, dim3(1), dim3(kThreads), lds, s, p, d->iters > 0 ? 1 : 0);
+    if (d->iters == 0) hipLaunchKernelGGL(k1_setup, dim3(1), dim3(kThreads), 0, s, p);
     if (d->iters > 0) {
         // img is Nx rows (x) of Ny values; At is Ny rows of Nx values
         hipLaunchKernelGGL(k1_transpose, dim3((p.Ny + 63) / 64, (p.Nx + 63) / 64), dim3(kThreads),
@@ -1038,13 +1094,11 @@ extern "C" int sarf_pga_scenes(sarf_ctx *ctx, int nscenes, const sarf_pga_scene 
     };
     std::vector<Group> groups;
     std::vector<int> grp;
-    int maxL = kMinL;
     for (int L = kMinL; L <= kMaxL; L++) {
         std::vector<int> ids;
         for (int i = 0; i < n; i++)
             if (host[i].L == L) ids.push_back(i);
         if (ids.empty()) continue;
-        maxL = L;
         Group g{L, (int)grp.size(), (int)ids.size(), 0};
         grp.insert(grp.end(), ids.begin(), ids.end());
         grp.push_back(0);
@@ -1098,8 +1152,7 @@ extern "C" int sarf_pga_scenes(sarf_ctx *ctx, int nscenes, const sarf_pga_scene 
         SARF_HIP(hipStreamSynchronize(s));
     }
     const auto total = [&](int kind) { return off[(size_t)kind * (n + 1) + n]; };
-    const size_t lds = sizeof(float2) * ((size_t)1 << maxL);
-    hipLaunchKernelGGL(kb_setup, dim3(n), dim3(kThreads), lds, s, b, iters > 0 ? 1 : 0);
+    if (iters == 0) hipLaunchKernelGGL(kb_setup, dim3(n), dim3(kThreads), 0, s, b);
     if (iters > 0) {
         hipLaunchKernelGGL(kb_transpose, dim3(total(kOffTranspose)), dim3(kThreads), 0, s, b, 1);
         for (int it = 0; it < iters; it++) {

@@ -543,12 +543,70 @@ int launch_jobs(const sarf_ctx *ctx, int rows, const RcJob *jobs, int njobs, int
     return launch_jobs_rows<LOG2N, 16>(ctx, jobs, njobs, ngroups, tw, s);
 }
 
+}  // namespace
+
+// Bluestein chirp and kernel spectrum of a length-N DFT with transform
+// length M = 2^L, in FP64: c_k = e^{+j pi k^2 / N} (k < N; the phase from the
+// exact k^2 mod 2N) and C = FFT_M(h) with h_d = conj(c_d) for d < N,
+// h_{M-d} = conj(c_d) for 0 < d < N, zero between (iterative radix 2 over
+// directly evaluated twiddles).
+void bluestein_f64(int N, int L, std::vector<double> &cr, std::vector<double> &ci,
+                   std::vector<double> &hr, std::vector<double> &hi) {
+    const int M = 1 << L;
+    cr.assign(N, 0.0);
+    ci.assign(N, 0.0);
+    for (int k = 0; k < N; k++) {
+        const long q = ((long)k * k) % (2L * N);
+        const double a = kPi * (double)q / (double)N;
+        cr[k] = std::cos(a);
+        ci[k] = std::sin(a);
+    }
+    hr.assign(M, 0.0);
+    hi.assign(M, 0.0);
+    for (int d = 0; d < N; d++) {
+        hr[d] = cr[d];
+        hi[d] = -ci[d];
+        if (d > 0) {
+            hr[M - d] = cr[d];
+            hi[M - d] = -ci[d];
+        }
+    }
+    // forward FFT (e^{-j 2 pi t / M}) over a table of the M/2 twiddles
+    std::vector<double> wr(M / 2 > 0 ? M / 2 : 1), wi(M / 2 > 0 ? M / 2 : 1);
+    for (int t = 0; t < M / 2; t++) {
+        const double a = -2.0 * kPi * (double)t / (double)M;
+        wr[t] = std::cos(a);
+        wi[t] = std::sin(a);
+    }
+    for (int i = 1, r = 0; i < M; i++) {
+        int bit = M >> 1;
+        for (; r & bit; bit >>= 1) r ^= bit;
+        r |= bit;
+        if (i < r) {
+            std::swap(hr[i], hr[r]);
+            std::swap(hi[i], hi[r]);
+        }
+    }
+    for (int len = 2; len <= M; len <<= 1) {
+        const int half = len / 2, step = M / len;
+        for (int st = 0; st < M; st += len)
+            for (int q = 0; q < half; q++) {
+                const double xr = wr[q * step], xi = wi[q * step];
+                const int i0 = st + q, i1 = i0 + half;
+                const double tr = hr[i1] * xr - hi[i1] * xi, ti = hr[i1] * xi + hi[i1] * xr;
+                hr[i1] = hr[i0] - tr;
+                hi[i1] = hi[i0] - ti;
+                hr[i0] += tr;
+                hi[i0] += ti;
+            }
+    }
+}
+
+namespace {
+
 // Bluestein tables of a length-N inverse DFT with transform length M = 2^L
-// (cached per N and precision in the context): chirp[k] = e^{+j pi k^2 / N},
-// k < N, and C = FFT_M(h) with h_d = conj(chirp[d]) for d < N,
-// h_{M-d} = conj(chirp[d]) for 0 < d < N, zero between; evaluated in FP64 (the
-// chirp phase from the exact k^2 mod 2N, the FFT with directly evaluated
-// twiddles) and rounded once to T2.  One allocation [chirp (N) | C (M)].
+// (cached per N and precision in the context): chirp (N) and C (M) of
+// bluestein_f64, rounded once to T2.  One allocation [chirp (N) | C (M)].
 template <typename T2>
 int bluestein_tables(sarf_ctx *ctx, int N, int L, const T2 **chirp, const T2 **C) {
     const int M = 1 << L;
@@ -556,52 +614,8 @@ int bluestein_tables(sarf_ctx *ctx, int N, int L, const T2 **chirp, const T2 **C
     std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->rc_blue.find(key);
     if (it == ctx->rc_blue.end()) {
-        std::vector<double> cr(N), ci(N);
-        for (int k = 0; k < N; k++) {
-            const long q = ((long)k * k) % (2L * N);
-            const double a = kPi * (double)q / (double)N;
-            cr[k] = std::cos(a);
-            ci[k] = std::sin(a);
-        }
-        std::vector<double> hr(M, 0.0), hi(M, 0.0);
-        for (int d = 0; d < N; d++) {
-            hr[d] = cr[d];
-            hi[d] = -ci[d];
-            if (d > 0) {
-                hr[M - d] = cr[d];
-                hi[M - d] = -ci[d];
-            }
-        }
-        // forward FFT (e^{-j 2 pi t / M}), iterative radix 2 over a table of
-        // the M/2 twiddles, each evaluated directly
-        std::vector<double> wr(M / 2 > 0 ? M / 2 : 1), wi(M / 2 > 0 ? M / 2 : 1);
-        for (int t = 0; t < M / 2; t++) {
-            const double a = -2.0 * kPi * (double)t / (double)M;
-            wr[t] = std::cos(a);
-            wi[t] = std::sin(a);
-        }
-        for (int i = 1, r = 0; i < M; i++) {
-            int bit = M >> 1;
-            for (; r & bit; bit >>= 1) r ^= bit;
-            r |= bit;
-            if (i < r) {
-                std::swap(hr[i], hr[r]);
-                std::swap(hi[i], hi[r]);
-            }
-        }
-        for (int len = 2; len <= M; len <<= 1) {
-            const int half = len / 2, step = M / len;
-            for (int st = 0; st < M; st += len)
-                for (int q = 0; q < half; q++) {
-                    const double xr = wr[q * step], xi = wi[q * step];
-                    const int i0 = st + q, i1 = i0 + half;
-                    const double tr = hr[i1] * xr - hi[i1] * xi, ti = hr[i1] * xi + hi[i1] * xr;
-                    hr[i1] = hr[i0] - tr;
-                    hi[i1] = hi[i0] - ti;
-                    hr[i0] += tr;
-                    hi[i0] += ti;
-                }
-        }
+        std::vector<double> cr, ci, hr, hi;
+        bluestein_f64(N, L, cr, ci, hr, hi);
         using Re = decltype(T2{}.x);
         std::vector<T2> h((size_t)N + M);
         for (int k = 0; k < N; k++) {
