@@ -456,6 +456,21 @@ def main() -> int:
     host_ms = (time.perf_counter() - t0) * 1e3 / 3
     log(f"bench: host-buffer path (sarf_focus, PCIe included) {host_ms:.2f} ms -> "
         f"{pp / (host_ms * 1e-3) / 1e9:.1f} Gpixel*pulses/s")
+    # the same with page-locked caller buffers (no staging copies)
+    ph_pin = torch.empty(tuple(ph.shape), dtype=torch.complex64, pin_memory=True)
+    ph_pin.copy_(ph)
+    img_pin = torch.empty((grid.Nx, grid.Ny), dtype=torch.complex64, pin_memory=True)
+    ph_pin_np, img_pin_np = ph_pin.numpy(), img_pin.numpy()
+    ctx.focus_host(ph_pin_np, geom.ant, geom.r0, geom.f0, grid, cfg.nfft, geom.maxWr,
+                   out=img_pin_np)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.focus_host(ph_pin_np, geom.ant, geom.r0, geom.f0, grid, cfg.nfft, geom.maxWr,
+                       out=img_pin_np)
+    host_pin_ms = (time.perf_counter() - t0) * 1e3 / 3
+    del ph_pin, img_pin, ph_pin_np, img_pin_np
+    log(f"bench: host-buffer path, page-locked buffers {host_pin_ms:.2f} ms -> "
+        f"{pp / (host_pin_ms * 1e-3) / 1e9:.1f} Gpixel*pulses/s")
 
     # roofline of the backprojection kernel
     achieved = FLOPS_PER_PP * pp / (t_bp * 1e-3) / 1e12
@@ -520,6 +535,9 @@ def main() -> int:
             "launch_shape": tune,
             "host_buffer_path_ms": round(host_ms, 3),
             "host_buffer_path_Gpixel_pulses_per_s": round(pp / (host_ms * 1e-3) / 1e9, 1),
+            "host_buffer_path_page_locked_ms": round(host_pin_ms, 3),
+            "host_buffer_path_page_locked_Gpixel_pulses_per_s":
+                round(pp / (host_pin_ms * 1e-3) / 1e9, 1),
         },
         "roofline": roof,
         "cpu_baseline": cpu,

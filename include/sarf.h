@@ -497,7 +497,9 @@ int sarf_pga_scenes(sarf_ctx *ctx, int nscenes, const sarf_pga_scene *scenes, in
 /* Whole focusing call on host buffers: H2D, range compression,             */
 /* backprojection, D2H.  This is what cuda_focus_SAR_image<float,float>     */
 /* (cuda_sar_focusing.hpp:526-723) does; it is the entry the C++ host       */
-/* wrappers in include/uncc_sar_focusing.hpp call.  Synchronous.            */
+/* wrappers in include/uncc_sar_focusing.hpp call.  Synchronous.  Pageable  */
+/* ph / img go through the context's pinned staging; page-locked ones       */
+/* (hipHostMalloc'd or registered) are copied directly.                     */
 /* ------------------------------------------------------------------------ */
 typedef struct {
     int K, Np, Nfft;

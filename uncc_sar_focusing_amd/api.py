@@ -443,8 +443,11 @@ class Context:
 
     # --------------------------------------------------------------- focus
     def focus_host(self, ph, ant, r0, f0, grid: ImageGrid, nfft: int, maxWr: float,
-                   precision: str = "mixed"):
-        """Whole RC + BP on host numpy arrays (what cuda_focus_SAR_image does)."""
+                   precision: str = "mixed", out=None):
+        """Whole RC + BP on host numpy arrays (what cuda_focus_SAR_image does).
+        `out` optionally gives the (Nx, Ny) complex64 image array to fill;
+        page-locked `ph` / `out` (e.g. numpy views of torch tensors made with
+        pin_memory=True) are copied to and from the GPU without staging."""
         import numpy as np
 
         ph = np.ascontiguousarray(ph, dtype=np.complex64)
@@ -452,7 +455,14 @@ class Context:
         ant = np.ascontiguousarray(ant, dtype=np.float64)
         r0 = np.ascontiguousarray(r0, dtype=np.float64)
         f0 = np.ascontiguousarray(f0, dtype=np.float64)
-        img = np.empty((grid.Nx, grid.Ny), dtype=np.complex64)
+        if out is None:
+            img = np.empty((grid.Nx, grid.Ny), dtype=np.complex64)
+        else:
+            if (out.dtype != np.complex64 or out.shape != (grid.Nx, grid.Ny)
+                    or not out.flags["C_CONTIGUOUS"]):
+                raise ValueError(f"out: need a contiguous complex64 array of shape "
+                                 f"{(grid.Nx, grid.Ny)}")
+            img = out
         d = FocusDesc(K, Np, nfft, grid.Nx, grid.Ny, grid.x0, grid.y0, grid.Wx, grid.Wy, maxWr,
                       _lib.SARF_BP_F64 if precision == "f64" else _lib.SARF_BP_MIXED)
         check(self.lib.sarf_focus(self._h, C.byref(d), ph.ctypes.data_as(_P),

@@ -174,6 +174,21 @@ int twiddles_f64(sarf_ctx *ctx, int log2n, const double2 **out) {
     return twiddles_any_f64(ctx, 1L << log2n, out);
 }
 
+// True if the host range [p, p + bytes) is page-locked (hipHostMalloc'd or
+// registered), so the GPU can copy from and to it directly.  For pageable
+// memory hipPointerGetAttributes fails; its error is cleared here.
+static bool host_pinned(const void *p, size_t bytes) {
+    for (const char *q : {(const char *)p, (const char *)p + (bytes ? bytes - 1 : 0)}) {
+        hipPointerAttribute_t a{};
+        if (hipPointerGetAttributes(&a, q) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        if (a.type != hipMemoryTypeHost) return false;
+    }
+    return true;
+}
+
 // Pinned staging of at least `bytes` bytes for the host-buffer path.
 int pinned_get(sarf_ctx *ctx, size_t bytes, char **out) {
     if (ctx->pinned_bytes < bytes) {
@@ -350,7 +365,8 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const d
         // while group g is range-compressed and backprojected; the image comes
         // back in chunks, each copied out of pinned staging while the next is
         // in flight.  Backprojection is additive over pulses, so each group
-        // accumulates into the image.
+        // accumulates into the image.  Page-locked caller buffers skip the
+        // staging: the copies go straight from and to them.
         // The first group's upload is not overlapped with anything, so it is
         // kept short (an eighth of the pulses); the rest split evenly.
         const int Gr = std::max(1, std::min(4, d->Np / 1024));
@@ -367,10 +383,17 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const d
             gmax = std::max(gmax, sizeof(float2) * (size_t)(n1 - n0) * d->K);
         }
         const size_t slot = (gmax + 255) & ~(size_t)255, img_bytes = img_elems * csz;
+        const bool ph_direct = host_pinned(ph, ph_elems * csz);
+        const bool img_direct = host_pinned(img, img_bytes);
         char *pin = nullptr;
-        SARF_TRY(pinned_get(ctx, 2 * slot + img_bytes, &pin));
-        char *pin_ph[2] = {pin, pin + slot};
-        char *pin_img = pin + 2 * slot;
+        const size_t stage = (ph_direct ? 0 : 2 * slot) + (img_direct ? 0 : img_bytes);
+        if (stage > 0) SARF_TRY(pinned_get(ctx, stage, &pin));
+        char *pin_ph[2] = {nullptr, nullptr}, *pin_img = pin;
+        if (!ph_direct) {
+            pin_ph[0] = pin;
+            pin_ph[1] = pin + slot;
+            pin_img = pin + 2 * slot;
+        }
         constexpr int kChunks = 4;
         hipStream_t sc = nullptr, s = nullptr;
         SARF_HIP(hipStreamCreateWithFlags(&sc, hipStreamNonBlocking));
@@ -390,13 +413,17 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const d
             const int n0 = group_start(g), n1 = group_start(g + 1);
             const size_t off = (size_t)n0 * d->K, bytes = sizeof(float2) * (size_t)(n1 - n0) * d->K;
             const int k = g & 1;
-            // staging slot k was last sent for group g - 2
-            if (g >= 2 && hipEventSynchronize(copied[k]) != hipSuccess) {
-                rc = fail(SARF_ERR_HIP, "sarf_focus: upload of group %d failed", g - 2);
-                break;
+            const void *src = (const float2 *)ph + off;
+            if (!ph_direct) {
+                // staging slot k was last sent for group g - 2
+                if (g >= 2 && hipEventSynchronize(copied[k]) != hipSuccess) {
+                    rc = fail(SARF_ERR_HIP, "sarf_focus: upload of group %d failed", g - 2);
+                    break;
+                }
+                ctx->copier->copy(pin_ph[k], src, bytes);
+                src = pin_ph[k];
             }
-            ctx->copier->copy(pin_ph[k], (const float2 *)ph + off, bytes);
-            if (hipMemcpyAsync((float2 *)dph + off, pin_ph[k], bytes, hipMemcpyHostToDevice, sc) !=
+            if (hipMemcpyAsync((float2 *)dph + off, src, bytes, hipMemcpyHostToDevice, sc) !=
                     hipSuccess ||
                 hipEventRecord(copied[k], sc) != hipSuccess ||
                 hipStreamWaitEvent(s, copied[k], 0) != hipSuccess) {
@@ -414,9 +441,10 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const d
             rc = launch_backproject(ctx, &bg, gprof, dant + 3 * (size_t)n0, dr0 + n0, df0 + n0,
                                     dimg, s);
         }
+        char *dst_img = img_direct ? (char *)img : pin_img;
         for (int c = 0; c < kChunks && rc == SARF_OK; c++) {
             const size_t b = img_bytes * c / kChunks, e = img_bytes * (c + 1) / kChunks;
-            if (hipMemcpyAsync(pin_img + b, (char *)dimg + b, e - b, hipMemcpyDeviceToHost, s) !=
+            if (hipMemcpyAsync(dst_img + b, (char *)dimg + b, e - b, hipMemcpyDeviceToHost, s) !=
                     hipSuccess ||
                 hipEventRecord(dl[c], s) != hipSuccess)
                 rc = fail(SARF_ERR_HIP, "sarf_focus: image download failed");
@@ -427,7 +455,7 @@ int sarf_focus(sarf_ctx *ctx, const sarf_focus_desc *d, const float *ph, const d
                 rc = fail(SARF_ERR_HIP, "sarf_focus: %s", hipGetErrorString(hipGetLastError()));
                 break;
             }
-            ctx->copier->copy((char *)img + b, pin_img + b, e - b);
+            if (!img_direct) ctx->copier->copy((char *)img + b, pin_img + b, e - b);
         }
         (void)hipStreamSynchronize(s);
         (void)hipStreamSynchronize(sc);
