@@ -158,17 +158,38 @@ __device__ void transpose_body(const float2 *__restrict__ src, int R, int C,
     }
 }
 
-// Row y: first maximum of |At[y][x]|^2 over x -> circular shift
-// s_y = ceil(N/2) - idx (mod N).
-__device__ void argmax_body(const PgaImage &p, int y) {
+// Rows of the argmax blocks: one wave per row.
+constexpr int kArgRows = kThreads / 64;
+__host__ __device__ constexpr int argmax_blocks(int Ny) { return (Ny + kArgRows - 1) / kArgRows; }
+
+// Argmax block blk, wave w: row y = blk * kArgRows + w, first maximum of
+// |At[y][x]|^2 over x -> circular shift s_y = ceil(N/2) - idx (mod N).  Each
+// lane scans x = lane, lane + 64, ...; the lanes then combine by butterflies
+// keeping the larger value, or the smaller index of equal values, so the
+// result is the row's first maximum whatever the lane order.  No barriers.
+__device__ void argmax_body(const PgaImage &p, int blk) {
     if (stopped(p)) return;
-    __shared__ float bm[kThreads];
-    __shared__ int bi[kThreads];
-    const int N = p.Nx, tid = threadIdx.x;
+    const int N = p.Nx, lane = threadIdx.x & 63;
+    const int y = blk * kArgRows + (threadIdx.x >> 6);
+    if (y >= p.Ny) return;
     const float2 *row = p.At + (size_t)y * N;
     float best = -1.f;
     int idx = 0;
-    for (int x = tid; x < N; x += kThreads) {
+    int x = lane;
+    for (; x + 192 < N; x += 256) {  // four loads in flight, compared in x order
+        float2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = row[x + 64 * u];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const float m = v[u].x * v[u].x + v[u].y * v[u].y;
+            if (m > best) {
+                best = m;
+                idx = x + 64 * u;
+            }
+        }
+    }
+    for (; x < N; x += 64) {
         const float2 v = row[x];
         const float m = v.x * v.x + v.y * v.y;
         if (m > best) {
@@ -176,23 +197,18 @@ __device__ void argmax_body(const PgaImage &p, int y) {
             idx = x;
         }
     }
-    bm[tid] = best;
-    bi[tid] = idx;
-    __syncthreads();
-    for (int o = kThreads / 2; o > 0; o >>= 1) {
-        if (tid < o) {
-            const float m2 = bm[tid + o];
-            const int i2 = bi[tid + o];
-            if (m2 > bm[tid] || (m2 == bm[tid] && i2 < bi[tid])) {
-                bm[tid] = m2;
-                bi[tid] = i2;
-            }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float m2 = __shfl_xor(best, o);
+        const int i2 = __shfl_xor(idx, o);
+        if (m2 > best || (m2 == best && i2 < idx)) {
+            best = m2;
+            idx = i2;
         }
-        __syncthreads();
     }
-    if (tid == 0) {
+    if (lane == 0) {
         const int center = (N + 1) / 2;  // 0-based target column = 1-based ceil(N/2)
-        p.shift[y] = ((center - bi[0]) % N + N) % N;
+        p.shift[y] = ((center - idx) % N + N) % N;
     }
 }
 
@@ -655,8 +671,8 @@ __device__ __forceinline__ void spectrum_regs(int N, const float2 *eph, const Sp
 }
 
 // Fused steps (fewer launches per iteration, same arithmetic):
-//   stats:     block b < Ny: argmax of row b; else a block of the column-energy
-//              partial sums (col_partial_body, mode 0);
+//   stats:     blocks b < argmax_blocks(Ny): argmax of kArgRows rows; then the
+//              blocks of the column-energy partial sums (col_partial_body, mode 0);
 //   window:    the column energy from its partials, then the window
 //              (one workgroup of kIntThreads);
 //   integrate: phi_dot from the phase-difference partials, the integration
@@ -664,10 +680,11 @@ __device__ __forceinline__ void spectrum_regs(int N, const float2 *eph, const Sp
 //              kernel's spectrum (one workgroup of kSpecThreads, the transforms
 //              in registers through a padded LDS row).
 __device__ void stats_body(const PgaImage &p, int b) {
-    if (b < p.Ny) {
+    const int nr = argmax_blocks(p.Ny);
+    if (b < nr) {
         argmax_body(p, b);
     } else {
-        const int l = b - p.Ny, gx = (p.Nx + 63) / 64;
+        const int l = b - nr, gx = (p.Nx + 63) / 64;
         col_partial_body(p, 0, l % gx, l / gx);
     }
 }
@@ -812,7 +829,7 @@ __global__ __launch_bounds__(kThreads) void kb_transpose(PgaBatchDev b, int forw
     if (forward) transpose_body(p.img, R, C, p.At, l % gx, l / gx);
     else transpose_body(p.At, R, C, p.img, l % gx, l / gx);
 }
-// Blocks [0, rows): argmax of every image's rows; then the column-energy
+// Blocks [0, rows): the argmax blocks of every image; then the column-energy
 // partial blocks of every image.
 __global__ __launch_bounds__(kThreads) void kb_stats(PgaBatchDev b, int rows) {
     if ((int)blockIdx.x < rows) {
@@ -1036,7 +1053,8 @@ extern "C" int sarf_pga(sarf_ctx *ctx, const sarf_pga_desc *d, void *img, double
                            0, s, p, 1);
         const dim3 gcp((p.Nx + 63) / 64, kRowChunks);
         for (int it = 0; it < d->iters; it++) {
-            hipLaunchKernelGGL(k1_stats, dim3(p.Ny + gcp.x * gcp.y), dim3(kThreads), 0, s, p);
+            hipLaunchKernelGGL(k1_stats, dim3(argmax_blocks(p.Ny) + gcp.x * gcp.y), dim3(kThreads),
+                               0, s, p);
             hipLaunchKernelGGL(k1_window, dim3(1), dim3(kIntThreads), 0, s, p, d->db_down);
             SARF_TRY(k1_row_step(p, 0, s));
             hipLaunchKernelGGL(k1_col_partial, gcp, dim3(kThreads), 0, s, p, 1);
@@ -1084,8 +1102,8 @@ extern "C" int sarf_pga_scenes(sarf_ctx *ctx, int nscenes, const sarf_pga_scene 
     std::vector<int> off((size_t)kOffKinds * (n + 1), 0);
     for (int i = 0; i < n; i++) {
         const PgaImage &p = host[i];
-        const int cnt[kOffKinds] = {((p.Nx + 63) / 64) * ((p.Ny + 63) / 64), p.Ny,
-                                    ((p.Nx + 63) / 64) * kRowChunks};
+        const int cnt[kOffKinds] = {((p.Nx + 63) / 64) * ((p.Ny + 63) / 64),
+                                    argmax_blocks(p.Ny), ((p.Nx + 63) / 64) * kRowChunks};
         for (int k = 0; k < kOffKinds; k++)
             off[(size_t)k * (n + 1) + i + 1] = off[(size_t)k * (n + 1) + i] + cnt[k];
     }
