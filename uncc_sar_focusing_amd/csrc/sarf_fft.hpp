@@ -132,40 +132,18 @@ struct Shape {
     static_assert(LOG2N >= 5, "at least one radix-16 pass after the first");
 };
 
-// Per-lane twiddles of the radix-16 passes of a length-2^LOG2N transform,
-// loaded ahead of it (they depend on the lane only):
-// w[pass] = tw[k * (N / (Ns * 16))], k = j mod Ns, Ns = R0 * 16^pass
-// (1 where k = 0, which the pass does not use).
-template <int LOG2N>
-struct PassTw {
-    float2 w[Shape<LOG2N>::NPASS16];
-};
-template <int LOG2N>
-__device__ __forceinline__ void load_pass_tw(const float2 *__restrict__ tw, int j,
-                                             PassTw<LOG2N> &t, bool active = true) {
-    using S = Shape<LOG2N>;
-    int Ns = S::R0;
-#pragma unroll
-    for (int pass = 0; pass < S::NPASS16; pass++) {
-        const int k = j & (Ns - 1);
-        t.w[pass] = (active && k != 0) ? tw[k * (S::N / (Ns * 16))] : make_float2(1.f, 0.f);
-        Ns *= 16;
-    }
-}
-
 // Unnormalised inverse transform (e^{+j 2 pi k m / N}) of one row by its T
 // lanes (lane j < T), through the row's padded LDS buffer L (padded(LOG2N)
-// float2); twf(pass, k, Ns) gives pass `pass`'s twiddle for k = j mod Ns,
-// tw[k * (N / (Ns * 16))] with tw[t] = e^{+j 2 pi t / N}.
+// float2).  tw[t] = e^{+j 2 pi t / N}.
 //   in:  v[b*R0 + r] = x[j + b*T + r*(N/R0)]  (b < NB0, r < R0)
 //   out: v[r]        = X[j + r*T]             (r < 16)
 // Starts with a barrier, so the caller may have just read L; every thread
 // of the workgroup must call it.  The forward transform is conj(ifft(conj)).
 // With active = false a thread only takes part in the barriers (for a
 // workgroup wider than the row's T lanes); its v is left unchanged.
-template <int LOG2N, class TwF>
-__device__ __forceinline__ void ifft_regs_tw(float2 (&v)[16], float2 *L, int j, TwF twf,
-                                             bool active) {
+template <int LOG2N>
+__device__ __forceinline__ void ifft_regs(float2 (&v)[16], float2 *L, int j,
+                                          const float2 *__restrict__ tw, bool active = true) {
     using S = Shape<LOG2N>;
     constexpr int N = S::N, T = S::T, R0 = S::R0, NB0 = S::NB0;
     __syncthreads();
@@ -189,7 +167,7 @@ __device__ __forceinline__ void ifft_regs_tw(float2 (&v)[16], float2 *L, int j, 
             for (int r = 0; r < 16; r++) v[r] = L[lpad(j + r * (N / 16))];
             if (k != 0) {
                 float2 p[16];
-                powers16(twf(pass, k, Ns), p);
+                powers16(tw[k * (N / (Ns * 16))], p);
 #pragma unroll
                 for (int r = 1; r < 16; r++) v[r] = cmul(v[r], p[r]);
             }
@@ -206,23 +184,6 @@ __device__ __forceinline__ void ifft_regs_tw(float2 (&v)[16], float2 *L, int j, 
         }
         Ns *= 16;
     }
-}
-
-// ifft_regs_tw reading each pass's twiddle from tw as the pass starts.
-template <int LOG2N>
-__device__ __forceinline__ void ifft_regs(float2 (&v)[16], float2 *L, int j,
-                                          const float2 *__restrict__ tw, bool active = true) {
-    constexpr int N = Shape<LOG2N>::N;
-    ifft_regs_tw<LOG2N>(
-        v, L, j, [tw](int, int k, int Ns) { return tw[k * (N / (Ns * 16))]; }, active);
-}
-
-// ifft_regs_tw with the twiddles loaded ahead (load_pass_tw).
-template <int LOG2N>
-__device__ __forceinline__ void ifft_regs(float2 (&v)[16], float2 *L, int j,
-                                          const PassTw<LOG2N> &t, bool active = true) {
-    ifft_regs_tw<LOG2N>(
-        v, L, j, [&t](int pass, int, int) { return t.w[pass]; }, active);
 }
 
 // Natural-order output of ifft_regs (u[r] = X[j + r*T]) rearranged into the

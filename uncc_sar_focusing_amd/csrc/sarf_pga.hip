@@ -572,101 +572,157 @@ __device__ __forceinline__ void row_apply_body(const PgaImage &p, int blk, float
 //   h = IDFT_N(e^{-j phi}) = conj(DFT_N(conj(e^{-j phi}))) / N   (Bluestein),
 //   b_m = h[(m - (N-1)) mod N] for m <= 2N-2, 0 up to M,  B = FFT_M(b),
 // so that IDFT_N(DFT_N(x) .* e^{-j phi}) = (x * b)[N-1 .. 2N-2] for a row x
-// zero-padded to M >= 2N-1.  The three length-M transforms run on the
-// register-resident radix-16 passes by the first M/16 lanes of the
-// workgroup (the others only join the barriers); Lr holds padded(L) float2.
+// zero-padded to M >= 2N-1.  It is one vector per image, formed by the one
+// integrate workgroup, so its three length-M transforms spread over all
+// kSpecThreads threads: self-sorting Stockham passes of radix 4 (one of
+// radix 2 first when log2 M is odd) through the LDS row, one to four
+// butterflies per thread and pass, twiddles from the sine/cosine unit.
 
-// The tables the spectrum reads, loaded by lane j < T = M/16 ahead of the
-// integration: win[b*R0 + r] = w_k, k = j + b*T + r*(M/R0) (the transform
-// input order), wout[r] = w_i and h[r] = H_i, i = j + r*T (its output
-// order), zero for k, i >= N; and the lane's pass twiddles.
-template <int L>
-struct SpecTabs {
-    float2 win[16], wout[16], h[16];
-    fft::PassTw<L> tw;
-};
-template <int L>
-__device__ __forceinline__ void load_spec_tabs(const PgaImage &p, SpecTabs<L> &t) {
-    using S = fft::Shape<L>;
-    constexpr int M = S::N, T = S::T, R0 = S::R0, NB0 = S::NB0;
-    const int j = threadIdx.x, N = p.Nx;
-    const bool act = j < T;
-    const float2 z = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int b = 0; b < NB0; b++)
-#pragma unroll
-        for (int r = 0; r < R0; r++) {
-            const int k = j + b * T + r * (M / R0);
-            t.win[b * R0 + r] = (act && k < N) ? p.wtab[k] : z;
-        }
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int i = j + r * T;
-        t.wout[r] = (act && i < N) ? p.wtab[i] : z;
-        t.h[r] = act ? p.H[i] : z;
-    }
-    fft::load_pass_tw<L>(p.twM, j, t.tw, act);
+// d * e^{j 2 pi rev}, rev in revolutions.
+__device__ __forceinline__ float2 rotate_rev(float2 d, float rev) {
+    const float c = __builtin_amdgcn_cosf(rev), s = __builtin_amdgcn_sinf(rev);
+    return make_float2(d.x * c - d.y * s, d.x * s + d.y * c);
 }
 
-// eph: e^{-j phi} as (cos phi, sin phi), N values in LDS.
-template <int L>
-__device__ __forceinline__ void spectrum_regs(int N, const float2 *eph, const SpecTabs<L> &t,
-                                              float2 *__restrict__ B, float2 *Lr) {
-    using S = fft::Shape<L>;
-    constexpr int M = S::N, T = S::T, R0 = S::R0, NB0 = S::NB0;
-    const int j = threadIdx.x;
-    const bool act = j < T;
-    float2 v[16];
-    // a_k = conj(e_k) w_k for k < N, conjugated: the forward transform is
-    // conj(ifft(conj))
+// One Stockham pass of radix R (2 or 4) and sign SIGN over buf (M values at
+// padded positions, fft::lpad) by NT threads, span Ns: butterfly b (k = b
+// mod Ns) reads x_r = buf[b + r M/R], rotates x_r by e^{SIGN j 2 pi r k /
+// (R Ns)}, takes the R-point DFT of sign SIGN and writes output q to
+// buf[(b - k) R + k + q Ns].  Every read precedes every write (barrier).
+template <int SIGN, int R, int M, int NT>
+__device__ __forceinline__ void stockham_pass(float2 *buf, int Ns) {
+    constexpr int NBF = M / R;
+    constexpr int PER = NBF > NT ? NBF / NT : 1;
+    static_assert(NBF <= NT || NBF % NT == 0, "butterflies per thread");
+    const int tid = threadIdx.x;
+    float2 x[PER][R];
 #pragma unroll
-    for (int b = 0; b < NB0; b++)
+    for (int q = 0; q < PER; q++) {
+        const int b = tid + q * NT;
+        if (b < NBF) {
 #pragma unroll
-        for (int r = 0; r < R0; r++) {
-            const int k = j + b * T + r * (M / R0);
-            float2 a = make_float2(0.f, 0.f);
-            if (act && k < N) a = cmul(eph[k], t.win[b * R0 + r]);
-            v[b * R0 + r] = conjf2(a);
-        }
-    fft::ifft_regs<L>(v, Lr, j, t.tw, act);  // v[r] = conj(A_k), k = j + r*T
-    float2 u[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) u[r] = act ? cmul(conjf2(v[r]), t.h[r]) : make_float2(0.f, 0.f);
-    fft::to_input_order<L>(u, v);
-    fft::ifft_regs<L>(v, Lr, j, t.tw, act);  // v[r] = c_k = M IFFT_M(A .* H)_k
-    __syncthreads();                           // every lane is done with Lr
-    // h_i = conj(w_i c_i) / (M N), i < N, staged in Lr
-    const float scale = 1.0f / ((float)M * (float)N);
-    if (act) {
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int i = j + r * T;
-            if (i < N) {
-                const float2 X = cmul(v[r], t.wout[r]);
-                Lr[i] = make_float2(X.x * scale, -X.y * scale);
-            }
+            for (int r = 0; r < R; r++) x[q][r] = buf[fft::lpad(b + r * NBF)];
         }
     }
     __syncthreads();
-    // b_m = h[(m - (N-1)) mod N] for m <= 2N-2, conjugated for the forward
-    // transform (ifft_regs starts with a barrier before it writes Lr)
 #pragma unroll
-    for (int b = 0; b < NB0; b++)
+    for (int q = 0; q < PER; q++) {
+        const int b = tid + q * NT;
+        if (b < NBF) {
+            const int k = b & (Ns - 1);
+            const float step = (float)(SIGN * k) / (float)(R * Ns);  // exact: powers of two
 #pragma unroll
-        for (int r = 0; r < R0; r++) {
-            const int m = j + b * T + r * (M / R0);
+            for (int r = 1; r < R; r++) x[q][r] = rotate_rev(x[q][r], step * (float)r);
+            float2 y[R];
+            if constexpr (R == 2) {
+                y[0] = cadd(x[q][0], x[q][1]);
+                y[1] = csub(x[q][0], x[q][1]);
+            } else {
+                const float2 s02 = cadd(x[q][0], x[q][2]), d02 = csub(x[q][0], x[q][2]);
+                const float2 s13 = cadd(x[q][1], x[q][3]), d13 = csub(x[q][1], x[q][3]);
+                // (SIGN j) * d13
+                const float2 jd = SIGN > 0 ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+                y[0] = cadd(s02, s13);
+                y[1] = cadd(d02, jd);
+                y[2] = csub(s02, s13);
+                y[3] = csub(d02, jd);
+            }
+            const int o = (b - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[fft::lpad(o + r * Ns)] = y[r];
+        }
+    }
+    __syncthreads();
+}
+
+// Unnormalised length-2^L transform of sign SIGN (e^{SIGN j 2 pi k m / M})
+// of buf, in place, by all NT threads of the workgroup; starts with a
+// barrier (the caller may just have written buf) and ends with one.
+template <int SIGN, int L, int NT>
+__device__ __forceinline__ void lds_transform(float2 *buf) {
+    constexpr int M = 1 << L;
+    __syncthreads();
+    int Ns = 1;
+    if constexpr ((L & 1) != 0) {
+        stockham_pass<SIGN, 2, M, NT>(buf, 1);
+        Ns = 2;
+    }
+#pragma unroll
+    for (int p = 0; p < L / 2; p++) {
+        stockham_pass<SIGN, 4, M, NT>(buf, Ns);
+        Ns *= 4;
+    }
+}
+
+// The spectrum's tables for the thread's positions k = tid + q * kSpecThreads
+// (q < M / kSpecThreads, at least one): w_k (k < N, else 0) and H_k, loaded
+// ahead of the integration.
+template <int L>
+struct SpecTabs {
+    static constexpr int Q = (1 << L) > kSpecThreads ? (1 << L) / kSpecThreads : 1;
+    float2 w[Q], h[Q];
+};
+template <int L>
+__device__ __forceinline__ void load_spec_tabs(const PgaImage &p, SpecTabs<L> &t) {
+    constexpr int M = 1 << L;
+#pragma unroll
+    for (int q = 0; q < SpecTabs<L>::Q; q++) {
+        const int k = threadIdx.x + q * kSpecThreads;
+        t.w[q] = k < p.Nx ? p.wtab[k] : make_float2(0.f, 0.f);
+        t.h[q] = k < M ? p.H[k] : make_float2(0.f, 0.f);
+    }
+}
+
+// eph: conj(e^{-j phi}) = (cos phi, sin phi), N values in LDS (overwritten
+// with h); buf: padded(L) float2 of LDS.  All kSpecThreads threads call it.
+template <int L>
+__device__ __forceinline__ void spectrum_lds(int N, float2 *eph, const SpecTabs<L> &t,
+                                             float2 *__restrict__ B, float2 *buf) {
+    constexpr int M = 1 << L, NT = kSpecThreads, Q = SpecTabs<L>::Q;
+    const int tid = threadIdx.x;
+    // a_k = conj(e_k) w_k for k < N, zero up to M
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int k = tid + q * NT;
+        if (k < M) buf[fft::lpad(k)] = k < N ? cmul(eph[k], t.w[q]) : make_float2(0.f, 0.f);
+    }
+    lds_transform<-1, L, NT>(buf);  // A = FFT_M(a)
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int k = tid + q * NT;
+        if (k < M) buf[fft::lpad(k)] = cmul(buf[fft::lpad(k)], t.h[q]);
+    }
+    lds_transform<+1, L, NT>(buf);  // c = M IFFT_M(A .* H)
+    // h_i = conj(w_i c_i) / (M N), i < N, staged in eph (read above)
+    const float scale = 1.0f / ((float)M * (float)N);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int i = tid + q * NT;
+        if (i < N) {
+            const float2 X = cmul(buf[fft::lpad(i)], t.w[q]);
+            eph[i] = make_float2(X.x * scale, -X.y * scale);
+        }
+    }
+    __syncthreads();
+    // b_m = h[(m - (N-1)) mod N] for m <= 2N-2, zero up to M
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int m = tid + q * NT;
+        if (m < M) {
             float2 bv = make_float2(0.f, 0.f);
-            if (act && m <= 2 * N - 2) {
+            if (m <= 2 * N - 2) {
                 int i = m - (N - 1);
                 if (i < 0) i += N;
-                bv = Lr[i];
+                bv = eph[i];
             }
-            v[b * R0 + r] = conjf2(bv);
+            buf[fft::lpad(m)] = bv;
         }
-    fft::ifft_regs<L>(v, Lr, j, t.tw, act);  // v[r] = conj(B_k)
-    if (act) {
+    }
+    lds_transform<-1, L, NT>(buf);  // B = FFT_M(b)
 #pragma unroll
-        for (int r = 0; r < 16; r++) B[j + r * T] = conjf2(v[r]);
+    for (int q = 0; q < Q; q++) {
+        const int k = tid + q * NT;
+        if (k < M) B[k] = buf[fft::lpad(k)];
     }
 }
 
@@ -726,7 +782,7 @@ __device__ __forceinline__ void integrate_l(const PgaImage &p, int it, float2 *L
     __syncthreads();
     if (!integrate_body(p, it, phid, af0, eph)) return;
     __syncthreads();
-    spectrum_regs<L>(N, eph, t, p.B, Lr);
+    spectrum_lds<L>(N, eph, t, p.B, Lr);
 }
 
 __device__ __forceinline__ void integrate_fused_body(const PgaImage &p, int it, float2 *Lr,
