@@ -75,7 +75,11 @@ const char *sarf_last_error(void);
  * captured: a caller that captures work of one context on several streams
  * orders them itself.  Scratch that has to grow is kept, not freed, until
  * sarf_ctx_trim or sarf_ctx_destroy, so work enqueued earlier (or a captured
- * graph) never refers to freed memory. */
+ * graph) never refers to freed memory.  Tables a transform length needs
+ * (twiddles, Bluestein chirps and their spectra) are built on its first use
+ * in the context, on the host with a synchronous upload, and kept until
+ * sarf_ctx_destroy: capture a call into a HIP graph after one eager call
+ * with the same sizes. */
 int sarf_ctx_create(int device, sarf_ctx **out);
 int sarf_ctx_destroy(sarf_ctx *ctx);
 
