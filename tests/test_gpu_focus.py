@@ -481,7 +481,9 @@ def test_bp_head_tail_split_matches_oracle(ctx, split):
 def test_bp_scenes_uneven_batch_matches_single_calls(ctx):
     """One persistent launch over scenes of different pulses, profile
     lengths and image sizes equals backprojecting each scene alone, and the
-    FP64 oracle within 1e-4."""
+    FP64 oracle within 1e-4.  A mixed batch (a tile per scene) is refused
+    here: at these coarse spacings none of its tiles 15, 16, 13 meets the
+    accuracy bound (the full-size C4 test runs it)."""
     specs = [(40, 64, 256, ImageGrid(37, 52, 0.1, -0.2, 4.0, 5.0)),
              (97, 96, 1024, ImageGrid.square(64, 0.1)),
              (16, 48, 512, ImageGrid(20, 90, 0.0, 0.3, 2.0, 9.0))]
@@ -501,7 +503,11 @@ def test_bp_scenes_uneven_batch_matches_single_calls(ctx):
             assert oracle.rel_l2(img, ref) <= 1e-4
             if tile == 0:
                 assert oracle.rel_l2(img, single) < 1e-5
+    with pytest.raises(_lib.SarfError) as e:
+        ctx.backproject_scenes(scenes, tile=_lib.SARF_SCENES_TILE_MIXED)
+    assert e.value.code == _lib.SARF_ERR_INVALID
     # the batch tuner returns an accurate tile whose batch meets the same bound
+    # (not the mixed mode, which these scenes cannot take)
     best, ms = ctx.autotune_scenes(scenes, reps=1)
     assert best in (1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16) and ms > 0
     imgs = ctx.backproject_scenes(scenes, tile=best)

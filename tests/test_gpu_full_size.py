@@ -30,6 +30,7 @@ import pytest
 import torch
 
 from scene_data import local_peaks, peak_index
+from uncc_sar_focusing_amd import _lib
 from uncc_sar_focusing_amd import autofocus as af
 from uncc_sar_focusing_amd.scenes import CONFIGS, grid_clutter
 
@@ -159,16 +160,32 @@ def test_c4_full_size_batch_matches_per_scene_paths(ctx):
         assert torch.equal(profs[i], ctx.range_compress(s.ph, s.nfft)), i
     descs = [(profs[i], s.ant, s.r0, s.f0, s.grid, s.geom.maxWr, None,
               s.geom.los(s.grid.x0, s.grid.y0)) for i, s in enumerate(scenes)]
+    mixed = ctx.backproject_scenes(descs, tile=_lib.SARF_SCENES_TILE_MIXED)
     imgs = ctx.backproject_scenes(descs)
-    worst = 0.0
+    worst = worst_mixed = 0.0
     for i, s in enumerate(scenes):
         p64 = ctx.range_compress(s.ph.to(torch.complex128), s.nfft)
         i64 = ctx.backproject(p64, s.ant, s.r0, s.f0, s.grid, s.geom.maxWr, precision="f64")
         worst = max(worst, _rel(imgs[i], i64))
+        worst_mixed = max(worst_mixed, _rel(mixed[i], i64))
         assert int(torch.argmax(imgs[i].abs())) == int(torch.argmax(i64.abs())), i
+        assert int(torch.argmax(mixed[i].abs())) == int(torch.argmax(i64.abs())), i
         del p64, i64
-    print(f"C4 batched BP worst rel-L2 vs FP64 over 64 scenes: {worst:.3e}")
-    assert worst <= 1e-4
+    print(f"C4 batched BP worst rel-L2 vs FP64 over 64 scenes: {worst:.3e} "
+          f"(a tile per scene: {worst_mixed:.3e})")
+    assert worst <= 1e-4 and worst_mixed <= 1e-4
+    # a mixed batch runs each scene on one of tiles 15, 16, 13 over all its
+    # pulses: every image equals the single-scene image of one of them
+    # without pulse splits (the same tile engine)
+    used = set()
+    for i, s in enumerate(scenes):
+        hits = [t for t in (15, 16, 13)
+                if torch.equal(mixed[i], ctx.backproject(profs[i], s.ant, s.r0, s.f0, s.grid,
+                                                         s.geom.maxWr, tile=t, pulse_split=1))]
+        assert hits, i
+        used.update(hits)
+    print(f"C4 mixed batch: tiles used {sorted(used)}")
+    del mixed
     big = max(range(len(scenes)), key=lambda i: scenes[i].pixel_pulses)
     sb = scenes[big]
     e_cpu, same_peak = _oracle_subset(ctx, sb.ph, profs[big], sb.ant, sb.r0, sb.f0, sb.grid,

@@ -24,6 +24,7 @@ SARF_BP_MIXED = 0
 SARF_BP_F64 = 1
 SARF_BP_ACCUMULATE = 1
 SARF_BATCH_DIR_PER_BASIS = 2
+SARF_SCENES_TILE_MIXED = 100
 SARF_RC_TAPER = 1
 SARF_RC_RAMP = 2
 SARF_RC_F64 = 4

@@ -225,6 +225,29 @@ bool bp_tile_accurate(const sarf_bp_desc *d, int id) {
     return lx * hx + ly * hy <= bound && 2.0 * (hx + hy) * invbin + 8 <= kBpCap;
 }
 
+// Of tiles 15, 16 and 13 (the lanes-along-x tiles of the default pick), the
+// accurate one with the lowest measured relative cost over the share of its
+// tile grid's pixels inside the image, as bp_pick_tile weighs them; 0 if
+// none is accurate.
+int bp_pick_xl_tile(const sarf_bp_desc *d) {
+    static const int kIds[] = {15, 16, 13};
+    static const double kCost[] = {1.0, 1.013, 1.05};
+    int best = 0;
+    double best_cost = 0.0;
+    for (int k = 0; k < 3; k++) {
+        int tx = 0, ty = 0;
+        if (!bp_tile_accurate(d, kIds[k]) || !bp_tile_shape(kIds[k], &tx, &ty)) continue;
+        const double ux = (double)d->Nx / (((d->Nx + tx - 1) / tx) * (double)tx);
+        const double uy = (double)d->Ny / (((d->Ny + ty - 1) / ty) * (double)ty);
+        const double cost = kCost[k] / (ux * uy);
+        if (best == 0 || cost < best_cost) {
+            best = kIds[k];
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
 // Returns the tile id if known (and its TX, TY), else 0.
 int bp_tile_shape(int id, int *tx, int *ty) {
     for (const TileShape &t : kTiles)

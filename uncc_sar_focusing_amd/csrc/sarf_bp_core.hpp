@@ -176,6 +176,7 @@ struct BpTile {
     static constexpr int TL = TY;                  // lanes along the lane axis
     static constexpr int GX = kBpThreads / TL;     // lane groups along the pixel axis
     static constexpr int NP = GX * PX;             // tile extent along the pixel axis
+    static constexpr int kPx = PX;                 // pixels per lane
     static constexpr int TX = XL ? TL : NP;        // tile extent along x
     static constexpr int TYE = XL ? NP : TL;       // tile extent along y
     static constexpr bool kSep = XL;               // separable local range

@@ -19,6 +19,7 @@
 namespace sarf {
 
 int bp_pick_tile(const sarf_bp_desc *d, double dx, double dy);
+int bp_pick_xl_tile(const sarf_bp_desc *d);
 int bp_tile_shape(int id, int *tx, int *ty);
 int bp_fill_args(sarf_ctx *ctx, const sarf_bp_desc *d, int tile, BpArgs *a);
 bool bp_tile_accurate(const sarf_bp_desc *d, int id);
@@ -30,7 +31,30 @@ struct SceneDev {
     const float2 *prof;
     const double *ant, *r0, *f0;
     float2 *img;
+    int shape;  // mixed batches: index of the scene's tile in kMixedIds
 };
+
+// Tiles of a mixed batch (all 2048 pixels, lanes along x) and their engines.
+constexpr int kMixedIds[3] = {15, 16, 13};
+using MixedT0 = BpTile<256, 8, true>;
+using MixedT1 = BpTile<128, 8, true>;
+using MixedT2 = BpTile<64, 8, true>;
+// One engine's LDS (staging and per-column terms); a mixed kernel holds
+// the union, no more than the largest engine alone.
+template <class Tile>
+struct EngineLds {
+    typename Tile::Lds L;
+    alignas(16) float fx[Tile::kFx];
+};
+union MixedLds {
+    EngineLds<MixedT0> e0;
+    EngineLds<MixedT1> e1;
+    EngineLds<MixedT2> e2;
+};
+// the three engines share one occupancy target
+static_assert(bp_waves_per_eu<256, 8, true>() == bp_waves_per_eu<128, 8, true>() &&
+                  bp_waves_per_eu<128, 8, true>() == bp_waves_per_eu<64, 8, true>(),
+              "mixed tiles: one waves-per-EU target");
 
 __global__ void reset_counter(unsigned int *c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
@@ -61,6 +85,46 @@ void bp_scenes(const SceneDev *__restrict__ scenes,
         for (int j = 0; j < PX; j++) {
             int x, y;
             if (T.pixel(j, x, y, a)) S.img[(size_t)x * a.Ny + y] = make_float2(T.accr[j], T.acci[j]);
+        }
+    }
+}
+
+// One work item of a scene: tile `tile` of the scene's image with tile
+// engine Tile.  Not inlined: each engine keeps its own register allocation
+// (inlined side by side, the three needed 170 VGPRs, half the occupancy).
+template <class Tile>
+__device__ __attribute__((noinline)) void scene_item(const SceneDev &S, int tile,
+                                                     typename Tile::Lds &L, float *fx) {
+    const BpArgs a = S.a;
+    Tile T;
+    T.init(a, tile);
+    T.run(a, L, fx, S.prof, PlainPos{S.ant}, S.r0, S.f0, 0, a.Np);
+#pragma unroll
+    for (int j = 0; j < Tile::kPx; j++) {
+        int x, y;
+        if (T.pixel(j, x, y, a)) S.img[(size_t)x * a.Ny + y] = make_float2(T.accr[j], T.acci[j]);
+    }
+}
+
+// bp_scenes with a tile shape per scene: an item runs its scene's engine
+// (uniform per workgroup); LDS is the union of the engines' own.
+__global__ __launch_bounds__(kBpThreads)
+__attribute__((amdgpu_waves_per_eu(bp_waves_per_eu<256, 8, true>()))) void bp_scenes_mixed(const SceneDev *__restrict__ scenes, const int2 *__restrict__ items,
+                     int nitems, unsigned int *__restrict__ counter) {
+    __shared__ MixedLds L;
+    __shared__ int sItem;
+    for (;;) {
+        __syncthreads();  // previous item done with sItem and L
+        if (threadIdx.x == 0) sItem = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int it = sItem;
+        if (it >= nitems) break;  // every workgroup reaches this once the list is drained
+        const int2 w = items[it];
+        const SceneDev &S = scenes[w.x];
+        switch (S.shape) {
+            case 0: scene_item<MixedT0>(S, w.y, L.e0.L, L.e0.fx); break;
+            case 1: scene_item<MixedT1>(S, w.y, L.e1.L, L.e1.fx); break;
+            default: scene_item<MixedT2>(S, w.y, L.e2.L, L.e2.fx); break;
         }
     }
 }
@@ -127,11 +191,17 @@ extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scen
         double cost;
     };
     std::vector<Item> items;
-    // one tile shape for the whole batch: the given one, else the default
-    int tid = tile > 0 ? tile : scenes_default_tile(nscenes, scenes);
-    int tx = 0, ty = 0;
-    if (!bp_tile_shape(tid, &tx, &ty) || !scene_tile_ok(tid))
-        return fail(SARF_ERR_INVALID, "sarf_bp_scenes: tile %d not available for batches", tid);
+    // one tile shape for the whole batch (the given one, else the default),
+    // or with SARF_SCENES_TILE_MIXED each scene's own among kMixedIds
+    const bool mixed = tile == SARF_SCENES_TILE_MIXED;
+    const int tid = mixed ? SARF_SCENES_TILE_MIXED
+                          : tile > 0 ? tile : scenes_default_tile(nscenes, scenes);
+    if (!mixed) {
+        int tx = 0, ty = 0;
+        if (!bp_tile_shape(tid, &tx, &ty) || !scene_tile_ok(tid))
+            return fail(SARF_ERR_INVALID, "sarf_bp_scenes: tile %d not available for batches",
+                        tid);
+    }
     for (int i = 0; i < nscenes; i++) {
         const sarf_scene &c = scenes[i];
         if (!c.prof || !c.ant || !c.r0 || !c.f0 || !c.img)
@@ -139,11 +209,22 @@ extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scen
         sarf_bp_desc d{};
         d.Nx = c.Nx; d.Ny = c.Ny; d.Nfft = c.Nfft; d.Np = c.Np;
         d.x0 = c.x0; d.y0 = c.y0; d.Wx = c.Wx; d.Wy = c.Wy; d.maxWr = c.maxWr;
+        d.los_x = c.los_x; d.los_y = c.los_y;
         d.precision = SARF_BP_MIXED;
+        int st = tid, shape = 0;
+        if (mixed) {
+            st = bp_pick_xl_tile(&d);
+            if (st == 0)
+                return fail(SARF_ERR_INVALID, "sarf_bp_scenes: scene %d has no accurate tile "
+                            "among 15, 16 and 13 for a mixed batch", i);
+            while (kMixedIds[shape] != st) shape++;
+        }
+        int tx = 0, ty = 0;
+        bp_tile_shape(st, &tx, &ty);
         BpArgs a{};
-        const int rc = bp_fill_args(ctx, &d, tid, &a);
+        const int rc = bp_fill_args(ctx, &d, st, &a);
         if (rc != SARF_OK) return fail(rc, "sarf_bp_scenes: scene %d: %s", i, sarf_last_error());
-        host[i] = SceneDev{a, (const float2 *)c.prof, c.ant, c.r0, c.f0, (float2 *)c.img};
+        host[i] = SceneDev{a, (const float2 *)c.prof, c.ant, c.r0, c.f0, (float2 *)c.img, shape};
         const int nt = a.tiles_x * a.tiles_y;
         for (int t = 0; t < nt; t++) items.push_back(Item{i, t, (double)c.Np * tx * ty});
     }
@@ -179,6 +260,11 @@ extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scen
     const int nit = (int)hitems.size();
     const int grid = std::min(nit, 4 * cus);
     switch (tid) {
+        case SARF_SCENES_TILE_MIXED:
+            hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, s, counter);
+            hipLaunchKernelGGL(bp_scenes_mixed, dim3(grid), dim3(kBpThreads), 0, s, dsc, ditems,
+                               nit, counter);
+            break;
         case 1: launch_scenes_t<32, 4>(grid, dsc, ditems, nit, counter, s); break;
         case 2: launch_scenes_t<64, 4>(grid, dsc, ditems, nit, counter, s); break;
         case 3: launch_scenes_t<64, 8>(grid, dsc, ditems, nit, counter, s); break;
@@ -195,10 +281,11 @@ extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scen
     return SARF_OK;
 }
 
-// Times sarf_bp_scenes with the default batch tile and every other batch
-// tile that meets the accuracy bound for all scenes, and returns the
-// fastest (1 % hysteresis towards the default, then the smaller id); waits
-// for the stream.  The images hold the last run's result.
+// Times sarf_bp_scenes with the default batch tile, every other batch tile
+// that meets the accuracy bound for all scenes and, if every scene has an
+// accurate one of its tiles, SARF_SCENES_TILE_MIXED, and returns the fastest
+// (1 % hysteresis towards the earlier candidate, in that order); waits for
+// the stream.  The images hold the last run's result.
 extern "C" int sarf_autotune_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scenes, int reps,
                                     int *best_tile, float *best_ms, void *stream) {
     clear_error();
@@ -227,6 +314,18 @@ extern "C" int sarf_autotune_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene
             ok = bp_tile_accurate(&d, id);
         }
         if (ok) cand.push_back(id);
+    }
+    {
+        bool ok = true;  // a tile per scene, if every scene has one
+        for (int i = 0; i < nscenes && ok; i++) {
+            const sarf_scene &c = scenes[i];
+            sarf_bp_desc d{};
+            d.Nx = c.Nx; d.Ny = c.Ny; d.Nfft = c.Nfft; d.Np = c.Np;
+            d.x0 = c.x0; d.y0 = c.y0; d.Wx = c.Wx; d.Wy = c.Wy; d.maxWr = c.maxWr;
+            d.los_x = c.los_x; d.los_y = c.los_y;
+            ok = bp_pick_xl_tile(&d) != 0;
+        }
+        if (ok) cand.push_back(SARF_SCENES_TILE_MIXED);
     }
     int best = 0;
     float bms = 0.f;
