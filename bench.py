@@ -713,6 +713,12 @@ def autofocus_leg(ctx, torch, levels: int = 5):
     return out
 
 
+def _lib_mixed_tile() -> int:
+    from uncc_sar_focusing_amd import _lib
+
+    return _lib.SARF_SCENES_TILE_MIXED
+
+
 def oracle_subset_check(ctx, torch, ph, prof, ant, r0, f0, grid, maxWr, nfft, los,
                         npulses: int = 128, first: int = 0):
     """The GPU image (range compression + mixed-precision backprojection) of
@@ -901,7 +907,8 @@ def serving_leg(ctx, torch, n_scenes: int = 64, pga_iters: int = 5, n_streams: i
         "rc": "batched (sarf_range_compress_scenes)",
         "batch_Gpixel_pulses_per_s": round(total_pp / (t_batch * 1e-3) / 1e9, 1),
         "batch_rc_bp_Gpixel_pulses_per_s": round(total_pp / (t_rcbp * 1e-3) / 1e9, 1),
-        "scenes_tile": scene_tile,
+        "scenes_tile": ("mixed: each scene on its cheapest of tiles 15, 16, 13"
+                        if scene_tile == _lib_mixed_tile() else scene_tile),
         "pga": "batched (sarf_pga_scenes)",
         "batch_per_image_pga_streams_gpu_ms": round(t_streams, 3),
         "pga_streams": n_streams,
