@@ -1,7 +1,8 @@
 """HIP graph capture of a serving batch: range compression of two uneven
 scenes (per scene, or batched in one sarf_range_compress_scenes call), their
-backprojection in one sarf_bp_scenes launch and PGA (per image, or batched
-in one sarf_pga_scenes call), captured
+backprojection in one sarf_bp_scenes launch (one tile for both, or a tile
+per scene) and PGA (per image, or batched in one sarf_pga_scenes call),
+captured
 once (after one eager call that uploads the scene list) and replayed
 several times with other GPU work and synchronisations in between.  Every
 replay must reproduce the eager result exactly (the kernels are
@@ -11,6 +12,7 @@ import pytest
 import torch
 
 from scene_data import small_inputs
+from uncc_sar_focusing_amd import _lib
 from uncc_sar_focusing_amd.scenes import ImageGrid
 
 pytestmark = pytest.mark.gpu
@@ -21,10 +23,16 @@ def _dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
 
 
-@pytest.mark.parametrize("mode", ["per_scene", "batched"])
+@pytest.mark.parametrize("mode", ["per_scene", "batched", "batched_mixed"])
 def test_serving_batch_graph_replays_match_eager(ctx, mode):
-    specs = [(48, 96, ImageGrid(70, 45, 0.3, -0.2, 6.9, 4.4), 1024),
-             (31, 64, ImageGrid(40, 52, -0.1, 0.2, 3.9, 5.1), 512)]
+    if mode == "batched_mixed":
+        # a tile per scene (fine spacings: tiles 15, 16, 13 are accurate)
+        specs = [(48, 96, ImageGrid(270, 45, 0.3, -0.2, 269 * 0.02, 44 * 0.02), 1024),
+                 (31, 64, ImageGrid(140, 52, -0.1, 0.2, 139 * 0.02, 51 * 0.02), 512)]
+    else:
+        specs = [(48, 96, ImageGrid(70, 45, 0.3, -0.2, 6.9, 4.4), 1024),
+                 (31, 64, ImageGrid(40, 52, -0.1, 0.2, 3.9, 5.1), 512)]
+    tile = _lib.SARF_SCENES_TILE_MIXED if mode == "batched_mixed" else 0
     scenes = []
     for Np, K, grid, nfft in specs:
         geom, tg, ph = small_inputs(Np=Np, K=K)
@@ -40,14 +48,14 @@ def test_serving_batch_graph_replays_match_eager(ctx, mode):
              torch.empty(2, dtype=torch.float64, device=DEV)) for s in scenes]
 
     def batch():
-        if mode == "batched":
+        if mode != "per_scene":
             ctx.range_compress_scenes([s["ph"] for s in scenes], [s["nfft"] for s in scenes],
                                       out=[s["prof"] for s in scenes])
         else:
             for s in scenes:
                 ctx.range_compress(s["ph"], s["nfft"], out=s["prof"])
-        ctx.backproject_scenes(descs)
-        if mode == "batched":
+        ctx.backproject_scenes(descs, tile=tile)
+        if mode != "per_scene":
             ctx.pga_scenes([s["img"] for s in scenes], 2, out=outs)
         else:
             for s in scenes:
