@@ -217,7 +217,10 @@ extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scen
             if (st == 0)
                 return fail(SARF_ERR_INVALID, "sarf_bp_scenes: scene %d has no accurate tile "
                             "among 15, 16 and 13 for a mixed batch", i);
-            while (kMixedIds[shape] != st) shape++;
+            while (shape < 3 && kMixedIds[shape] != st) shape++;
+            if (shape == 3)
+                return fail(SARF_ERR_INVALID, "sarf_bp_scenes: scene %d: tile %d has no engine in "
+                            "a mixed batch", i, st);
         }
         int tx = 0, ty = 0;
         bp_tile_shape(st, &tx, &ty);
