@@ -418,13 +418,15 @@ int sarf_bp_entropy_grad(sarf_ctx *ctx, const sarf_bp_desc *desc,
 /* persistent launch (work items = scene tiles, largest first, drained      */
 /* through an atomic counter).  Each scene is described as for              */
 /* sarf_backproject, with its own device buffers; images are overwritten.   */
-/* tile: 0 = chosen for the finest-spaced scene, else 1..5 or 11..16 for   */
-/* every scene, or SARF_SCENES_TILE_MIXED: each scene on the one of tiles  */
-/* 15, 16 and 13 that sarf_backproject's default weighs cheapest for it     */
-/* (relative speed over the share of its tile grid inside the image), all  */
-/* in the same launch; SARF_ERR_INVALID if a scene has no accurate tile     */
-/* among the three.  Tiles 15, 16 and 13 run lanes along x, the fast        */
-/* orientation for a line of sight along y.                                 */
+/* tile: 1..5 or 11..16 for every scene, or SARF_SCENES_TILE_MIXED: each   */
+/* scene on the one of tiles 15, 16 and 13 that sarf_backproject's default  */
+/* weighs cheapest for it (relative speed over the share of its tile grid   */
+/* inside the image), all in the same launch; SARF_ERR_INVALID if a scene   */
+/* has no accurate tile among the three.  Tiles 15, 16 and 13 run lanes     */
+/* along x, the fast orientation for a line of sight along y.  0: a tile    */
+/* per scene when every scene has a line-of-sight hint with |los_y| >=      */
+/* |los_x| and an accurate tile among the three, else one tile for all,     */
+/* chosen for the finest-spaced scene.                                      */
 /* Each distinct work list (descriptors and tile) is uploaded once (the     */
 /* call waits for the copy) into device memory of its own, kept until       */
 /* sarf_ctx_trim or destroy; a list seen before only enqueues kernels, so   */

@@ -365,9 +365,11 @@ class Context:
         """Backproject many independent scenes in one persistent launch.
         `scenes` is a sequence of (prof, ant, r0, f0, grid, maxWr[, img[, los]])
         with img None or omitted to allocate it and los an optional
-        (los_x, los_y) tile hint; returns the images.  `tile`: 0 (default
-        choice), one tile id for every scene, or _lib.SARF_SCENES_TILE_MIXED
-        (each scene on its cheapest of tiles 15, 16, 13)."""
+        (los_x, los_y) tile hint; returns the images.  `tile`: 0 (default:
+        a tile per scene when every scene has a hint along y and can take
+        one, else one tile for all), one tile id for every scene, or
+        _lib.SARF_SCENES_TILE_MIXED (each scene on its cheapest of tiles 15,
+        16, 13)."""
         descs, keep = self._scene_descs(scenes)
         check(self.lib.sarf_bp_scenes(self._h, len(scenes), descs, tile, self._stream()))
         return keep

@@ -176,6 +176,23 @@ int scenes_default_tile(int nscenes, const sarf_scene *scenes) {
     return tid;
 }
 
+// True if every scene can take a tile of its own among 15, 16 and 13 and
+// has a line-of-sight hint along y (|los_y| >= |los_x|), the orientation
+// those lanes-along-x tiles are fast for.
+bool scenes_mixed_ok(int nscenes, const sarf_scene *scenes, bool need_hint) {
+    for (int i = 0; i < nscenes; i++) {
+        const sarf_scene &c = scenes[i];
+        if (need_hint && !(c.los_x != 0.0 || c.los_y != 0.0)) return false;
+        if (need_hint && std::fabs(c.los_y) < std::fabs(c.los_x)) return false;
+        sarf_bp_desc d{};
+        d.Nx = c.Nx; d.Ny = c.Ny; d.Nfft = c.Nfft; d.Np = c.Np;
+        d.x0 = c.x0; d.y0 = c.y0; d.Wx = c.Wx; d.Wy = c.Wy; d.maxWr = c.maxWr;
+        d.los_x = c.los_x; d.los_y = c.los_y;
+        if (bp_pick_xl_tile(&d) == 0) return false;
+    }
+    return true;
+}
+
 }  // namespace
 }  // namespace sarf
 
@@ -193,7 +210,9 @@ extern "C" int sarf_bp_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene *scen
     std::vector<Item> items;
     // one tile shape for the whole batch (the given one, else the default),
     // or with SARF_SCENES_TILE_MIXED each scene's own among kMixedIds
-    const bool mixed = tile == SARF_SCENES_TILE_MIXED;
+    // default: a tile per scene when every scene suits it, else one tile
+    const bool mixed = tile == SARF_SCENES_TILE_MIXED ||
+                       (tile == 0 && scenes_mixed_ok(nscenes, scenes, true));
     const int tid = mixed ? SARF_SCENES_TILE_MIXED
                           : tile > 0 ? tile : scenes_default_tile(nscenes, scenes);
     if (!mixed) {
@@ -303,7 +322,8 @@ extern "C" int sarf_autotune_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene
         (void)hipEventDestroy(e0);
         return fail(SARF_ERR_HIP, "sarf_autotune_scenes: event creation failed");
     }
-    const int def = scenes_default_tile(nscenes, scenes);
+    const int def = scenes_mixed_ok(nscenes, scenes, true) ? SARF_SCENES_TILE_MIXED
+                                                            : scenes_default_tile(nscenes, scenes);
     std::vector<int> cand{def};
     for (int id : kSceneTiles) {
         if (id == def) continue;
@@ -318,18 +338,9 @@ extern "C" int sarf_autotune_scenes(sarf_ctx *ctx, int nscenes, const sarf_scene
         }
         if (ok) cand.push_back(id);
     }
-    {
-        bool ok = true;  // a tile per scene, if every scene has one
-        for (int i = 0; i < nscenes && ok; i++) {
-            const sarf_scene &c = scenes[i];
-            sarf_bp_desc d{};
-            d.Nx = c.Nx; d.Ny = c.Ny; d.Nfft = c.Nfft; d.Np = c.Np;
-            d.x0 = c.x0; d.y0 = c.y0; d.Wx = c.Wx; d.Wy = c.Wy; d.maxWr = c.maxWr;
-            d.los_x = c.los_x; d.los_y = c.los_y;
-            ok = bp_pick_xl_tile(&d) != 0;
-        }
-        if (ok) cand.push_back(SARF_SCENES_TILE_MIXED);
-    }
+    // a tile per scene, if every scene has one
+    if (def != SARF_SCENES_TILE_MIXED && scenes_mixed_ok(nscenes, scenes, false))
+        cand.push_back(SARF_SCENES_TILE_MIXED);
     int best = 0;
     float bms = 0.f;
     int rc = SARF_OK;
