@@ -98,6 +98,17 @@ __device__ __forceinline__ void dft(float2 *v) {
 
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
 
+// e^{j 2 pi r rev} for r = 1..15 from the sine/cosine unit, which takes
+// revolutions (rev = k / (Ns 16) is exact, so each twiddle is one rounding
+// of the unit, about 1e-7).
+__device__ __forceinline__ void powers16_rev(float rev, float2 *p) {
+#pragma unroll
+    for (int r = 1; r < 16; r++) {
+        const float a = rev * (float)r;
+        p[r] = make_float2(__builtin_amdgcn_cosf(a), __builtin_amdgcn_sinf(a));
+    }
+}
+
 // w^r for r = 0..15 from w (r = 0 unused); error <= ~4 roundings.
 __device__ __forceinline__ void powers16(float2 w, float2 *p) {
     p[1] = w;
@@ -110,6 +121,21 @@ __device__ __forceinline__ void powers16(float2 w, float2 *p) {
     p[8] = cmul(p[4], p[4]);
 #pragma unroll
     for (int r = 9; r < 16; r++) p[r] = cmul(p[8], p[r - 8]);
+}
+
+// Pass twiddles of a length-2^LOG2N transform for k = j mod Ns: from the
+// sine/cosine unit up to 2^13 (measured faster there, and more accurate:
+// range compression rel-L2 1.5-3e-7 against 3-5e-7), from the table entry
+// w = tw[k N / (Ns 16)] and its products (powers16) at 2^14 and above.
+template <int LOG2N>
+__device__ __forceinline__ void pass_twiddles(const float2 *__restrict__ tw, int k, int Ns,
+                                              float2 *p) {
+    constexpr int N = 1 << LOG2N;
+    if constexpr (LOG2N <= 13) {
+        powers16_rev((float)k / (float)(Ns * 16), p);
+    } else {
+        powers16(tw[k * (N / (Ns * 16))], p);
+    }
 }
 
 
@@ -167,7 +193,7 @@ __device__ __forceinline__ void ifft_regs(float2 (&v)[16], float2 *L, int j,
             for (int r = 0; r < 16; r++) v[r] = L[lpad(j + r * (N / 16))];
             if (k != 0) {
                 float2 p[16];
-                powers16(tw[k * (N / (Ns * 16))], p);
+                pass_twiddles<LOG2N>(tw, k, Ns, p);
 #pragma unroll
                 for (int r = 1; r < 16; r++) v[r] = cmul(v[r], p[r]);
             }

@@ -36,7 +36,7 @@ namespace {
 
 using fft::dft;
 using fft::lpad;
-using fft::powers16;
+using fft::pass_twiddles;
 
 // Longest non-power-of-two transform done by the register-resident
 // Bluestein kernel (M <= 16384); longer ones (and FP64) go through the
@@ -193,7 +193,7 @@ __device__ __forceinline__ void rc_lds_body(const Jobs &J, int ngroups,
             const int k = jj & (Ns - 1);
             if (k != 0) {
                 float2 p[16];
-                powers16(tw[k * (N / (Ns * 16))], p);
+                pass_twiddles<LOG2N>(tw, k, Ns, p);
 #pragma unroll
                 for (int r = 1; r < 16; r++) v[r] = cmul(v[r], p[r]);
             }
