@@ -123,10 +123,13 @@ __device__ __forceinline__ void powers16(float2 w, float2 *p) {
     for (int r = 9; r < 16; r++) p[r] = cmul(p[8], p[r - 8]);
 }
 
-// Pass twiddles of a length-2^LOG2N transform for k = j mod Ns: from the
-// sine/cosine unit up to 2^13 (measured faster there, and more accurate:
-// range compression rel-L2 1.5-3e-7 against 3-5e-7), from the table entry
-// w = tw[k N / (Ns 16)] and its products (powers16) at 2^14 and above.
+// Pass twiddles of a length-2^LOG2N transform for k = j mod Ns.  Up to 2^13
+// from the sine/cosine unit: measured on MI355X, a batch of range
+// compressions at 2^11-2^13 points ran 7 % faster and single ones as fast,
+// with rel-L2 against FP64 of 1.5-2.2e-7 instead of 2.9-3.6e-7.  At 2^14 and
+// above from the table entry w = tw[k N / (Ns 16)] and its products
+// (powers16), which measured faster there (2^14: 0.191 against 0.196 ms for
+// 4096 x 2048 -> 16384).
 template <int LOG2N>
 __device__ __forceinline__ void pass_twiddles(const float2 *__restrict__ tw, int k, int Ns,
                                               float2 *p) {
