@@ -228,7 +228,28 @@ __device__ void col_partial_body(const PgaImage &p, int mode, int bx, int chunk)
     const int yb = chunk * per, ye = min(Ny, yb + per);
     double a = 0.0, b = 0.0;
     if (x < N) {
-        for (int y = yb + ly; y < ye; y += 4) {
+        // rows y = yb + ly, yb + ly + 4, ... added in that order, four rows'
+        // loads issued before any is consumed
+        const int x1 = x + 1 < N ? x + 1 : x;
+        int y = yb + ly;
+        for (; y + 12 < ye; y += 16) {
+            float2 g[4], g1[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                g[u] = src[(size_t)(y + 4 * u) * N + x];
+                if (mode == 1) g1[u] = src[(size_t)(y + 4 * u) * N + x1];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (mode == 0) {
+                    a += (double)g[u].x * g[u].x + (double)g[u].y * g[u].y;
+                } else if (x + 1 < N) {
+                    a += (double)g[u].x * g1[u].y - (double)g[u].y * g1[u].x;
+                    b += (double)g[u].x * g[u].x + (double)g[u].y * g[u].y;
+                }
+            }
+        }
+        for (; y < ye; y += 4) {
             const float2 g = src[(size_t)y * N + x];
             if (mode == 0) {
                 a += (double)g.x * g.x + (double)g.y * g.y;
