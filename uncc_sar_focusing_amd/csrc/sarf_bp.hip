@@ -63,7 +63,19 @@ __global__ __launch_bounds__(256) void bp_reduce_splits(const float2 *__restrict
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float2 acc = accumulate ? img[i] : make_float2(0.f, 0.f);
-    for (int s = 0; s < splits; s++) {
+    // partials added in split order; eight loads issued before any is added
+    int s = 0;
+    for (; s + 8 <= splits; s += 8) {
+        float2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = part[(s + u) * stride + i];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            acc.x += v[u].x;
+            acc.y += v[u].y;
+        }
+    }
+    for (; s < splits; s++) {
         const float2 v = part[s * stride + i];
         acc.x += v.x;
         acc.y += v.y;

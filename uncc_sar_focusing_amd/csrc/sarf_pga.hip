@@ -145,15 +145,27 @@ __device__ void init_body(const PgaImage &p) {
 __device__ void transpose_body(const float2 *__restrict__ src, int R, int C,
                                float2 *__restrict__ dst, int bx, int by) {
     __shared__ float2 tile[64][65];
+    constexpr int kStep = kThreads / 64, kRows = 64 / kStep;  // rows i = ly + k kStep
     const int r0 = by * 64, c0 = bx * 64;
     const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
-    for (int i = ly; i < 64; i += kThreads / 64) {
-        const int r = r0 + i, c = c0 + lx;
-        if (r < R && c < C) tile[i][lx] = src[(size_t)r * C + c];
+    {
+        // every load of the thread issued before the first is consumed
+        // (a loop with the LDS store in it keeps one load in flight)
+        const int c = c0 + lx;
+        float2 v[kRows];
+#pragma unroll
+        for (int k = 0; k < kRows; k++) {
+            const int r = r0 + ly + k * kStep;
+            v[k] = (r < R && c < C) ? src[(size_t)r * C + c] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int k = 0; k < kRows; k++) tile[ly + k * kStep][lx] = v[k];
     }
     __syncthreads();
-    for (int i = ly; i < 64; i += kThreads / 64) {
-        const int c = c0 + i, r = r0 + lx;
+    const int r = r0 + lx;
+#pragma unroll
+    for (int k = 0; k < kRows; k++) {
+        const int i = ly + k * kStep, c = c0 + i;
         if (r < R && c < C) dst[(size_t)c * R + r] = tile[lx][i];
     }
 }
